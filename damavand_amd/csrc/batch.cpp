@@ -1,0 +1,272 @@
+// Batched forward + adjoint of many small circuits in one persistent launch,
+// and the autotuner of the fused tile shape.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "device.hpp"
+#include "kernels.hpp"
+#include "runtime.hpp"
+
+namespace dmv {
+
+void tune_set(const char *name, int64_t v);
+
+int batch_forward_adjoint(int count, int dtype, const int *n, const int *gate_offset, const int *gate_kind,
+                          const int *gate_control, const int *gate_target, const double *gate_param,
+                          const int *coef_offset, const double *coef, const int *param_offset, double *energy,
+                          double *grad)
+{
+    DMV_CHECK_ARG(count >= 1, "empty batch");
+    DMV_CHECK_ARG(dtype == DMV_C128 || dtype == DMV_C64, "bad dtype");
+    DMV_CHECK_ARG(n && gate_offset && gate_kind && gate_control && gate_target && gate_param && coef_offset && coef &&
+                      param_offset && energy && grad,
+                  "null batch array");
+    DMV_TRY(ensure_device());
+    int rlog, tlog;
+    batch_tile_shape(dtype, &rlog, &tlog);
+    const int K = rlog + tlog;
+    const int small_max = K - 1;
+
+    std::vector<BatchCircuit> circ(count);
+    std::vector<DevPass> passes;
+    std::vector<DevPhase> phases;
+    std::vector<DevOp> ops, sops;
+    std::vector<double> cost(count);
+    int nmax = 1;
+    int64_t total_params = 0;
+    for (int i = 0; i < count; i++) {
+        const int ni = n[i];
+        DMV_CHECK_ARG(ni >= 2 && ni <= 24, "batch circuit %d: qubits must be in [2, 24], got %d", i, ni);
+        nmax = std::max(nmax, ni);
+        std::vector<Gate> gates;
+        for (int g = gate_offset[i]; g < gate_offset[i + 1]; g++) {
+            Gate G;
+            G.kind = gate_kind[g];
+            G.control = gate_control[g];
+            G.target = gate_target[g];
+            G.p[0] = gate_param[3 * (int64_t)g];
+            G.p[1] = gate_param[3 * (int64_t)g + 1];
+            G.p[2] = gate_param[3 * (int64_t)g + 2];
+            G.observable = false;
+            DMV_CHECK_ARG(G.kind >= DMV_GATE_H && G.kind <= DMV_GATE_CZ, "batch circuit %d: bad gate kind", i);
+            DMV_CHECK_ARG(G.target >= 0 && G.target < ni, "batch circuit %d: target out of range", i);
+            const bool ctl = G.kind == DMV_GATE_CNOT || G.kind == DMV_GATE_CZ;
+            DMV_CHECK_ARG(!ctl || (G.control >= 0 && G.control < ni && G.control != G.target),
+                          "batch circuit %d: bad control", i);
+            if (!ctl)
+                G.control = -1;
+            gates.push_back(G);
+        }
+        int np = 0;
+        for (const Gate &G : gates)
+            if (gate_is_parametric(G.kind))
+                np++;
+        total_params += np;
+        std::vector<HostOp> f = merge_ops(forward_ops(gates));
+        std::vector<HostOp> a = merge_ops(adjoint_ops(gates));
+        BatchCircuit &C = circ[i];
+        C.n = (uint32_t)ni;
+        C.coef = (uint32_t)coef_offset[i];
+        C.grad = (uint32_t)param_offset[i];
+        cost[i] = (double)(f.size() + a.size()) * std::ldexp(1.0, ni);
+        if (ni <= small_max) {
+            C.small = 1;
+            std::vector<DevOp> fo = global_dev_ops(f), ao = global_dev_ops(a);
+            C.fwd_begin = (uint32_t)sops.size();
+            sops.insert(sops.end(), fo.begin(), fo.end());
+            C.fwd_end = (uint32_t)sops.size();
+            C.adj_begin = (uint32_t)sops.size();
+            sops.insert(sops.end(), ao.begin(), ao.end());
+            C.adj_end = (uint32_t)sops.size();
+        } else {
+            C.small = 0;
+            PlanParams pp;
+            pp.n = ni;
+            pp.k = K;
+            pp.rlog = rlog;
+            pp.tlog = tlog;
+            pp.low_bits = coalesce_bits(dtype);
+            pp.swz_bits = dtype == DMV_C64 ? 4 : 3;
+            pp.max_gen_per_pass = (int)tune_value("max_gen_per_pass", 24);
+            for (int which = 0; which < 2; which++) {
+                Plan plan;
+                DMV_TRY(build_plan(which == 0 ? f : a, pp, &plan));
+                const uint32_t pbase = (uint32_t)passes.size();
+                const uint32_t phbase = (uint32_t)phases.size();
+                const uint32_t obase = (uint32_t)ops.size();
+                for (DevPass P : plan.passes) {
+                    P.phase_begin += phbase;
+                    P.phase_end += phbase;
+                    passes.push_back(P);
+                }
+                for (DevPhase D : plan.phases) {
+                    D.op_begin += obase;
+                    D.op_end += obase;
+                    phases.push_back(D);
+                }
+                ops.insert(ops.end(), plan.ops.begin(), plan.ops.end());
+                if (which == 0) {
+                    C.fwd_begin = pbase;
+                    C.fwd_end = (uint32_t)passes.size();
+                } else {
+                    C.adj_begin = pbase;
+                    C.adj_end = (uint32_t)passes.size();
+                }
+            }
+        }
+        DMV_CHECK_ARG(coef_offset[i] >= 0, "bad coefficient offset");
+    }
+    std::vector<uint32_t> order(count);
+    for (int i = 0; i < count; i++)
+        order[i] = (uint32_t)i;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return cost[x] > cost[y]; });
+
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, 0) != hipSuccess) {
+        set_error("hipGetDeviceProperties failed");
+        return DMV_ERR_HIP;
+    }
+    const int per_cu = (int)tune_value("batch_wgs_per_cu", 2);
+    uint32_t grid = (uint32_t)std::min<int64_t>(count, (int64_t)prop.multiProcessorCount * std::max(1, per_cu));
+    const uint64_t slot_amps = uint64_t(1) << std::max(nmax, K);
+    const uint64_t scratch_bytes = (uint64_t)grid * 2 * slot_amps * amp_bytes(dtype);
+
+    int64_t coef_len = 0;
+    for (int i = 0; i < count; i++)
+        coef_len = std::max<int64_t>(coef_len, (int64_t)coef_offset[i] + 2 * n[i] - 1);
+
+    // device buffers
+    struct Buf {
+        void *p = nullptr;
+        ~Buf() { release(p); }
+    } b_circ, b_order, b_counter, b_passes, b_phases, b_ops, b_sops, b_coef, b_scratch, b_energy, b_grad;
+    DMV_TRY(alloc(&b_circ.p, circ.size() * sizeof(BatchCircuit)));
+    DMV_TRY(alloc(&b_order.p, order.size() * sizeof(uint32_t)));
+    DMV_TRY(alloc(&b_counter.p, 64));
+    DMV_TRY(alloc(&b_passes.p, std::max<size_t>(1, passes.size()) * sizeof(DevPass)));
+    DMV_TRY(alloc(&b_phases.p, std::max<size_t>(1, phases.size()) * sizeof(DevPhase)));
+    DMV_TRY(alloc(&b_ops.p, std::max<size_t>(1, ops.size()) * sizeof(DevOp)));
+    DMV_TRY(alloc(&b_sops.p, std::max<size_t>(1, sops.size()) * sizeof(DevOp)));
+    DMV_TRY(alloc(&b_coef.p, (uint64_t)std::max<int64_t>(1, coef_len) * sizeof(double)));
+    DMV_TRY(alloc(&b_scratch.p, scratch_bytes));
+    DMV_TRY(alloc(&b_energy.p, (uint64_t)count * sizeof(double)));
+    DMV_TRY(alloc(&b_grad.p, (uint64_t)std::max<int64_t>(1, total_params) * sizeof(double)));
+    DMV_TRY(copy_h2d(b_circ.p, circ.data(), circ.size() * sizeof(BatchCircuit)));
+    DMV_TRY(copy_h2d(b_order.p, order.data(), order.size() * sizeof(uint32_t)));
+    DMV_TRY(copy_h2d(b_passes.p, passes.data(), passes.size() * sizeof(DevPass)));
+    DMV_TRY(copy_h2d(b_phases.p, phases.data(), phases.size() * sizeof(DevPhase)));
+    DMV_TRY(copy_h2d(b_ops.p, ops.data(), ops.size() * sizeof(DevOp)));
+    DMV_TRY(copy_h2d(b_sops.p, sops.data(), sops.size() * sizeof(DevOp)));
+    DMV_TRY(copy_h2d(b_coef.p, coef, (uint64_t)coef_len * sizeof(double)));
+
+    BatchLaunch L;
+    L.circ = (const BatchCircuit *)b_circ.p;
+    L.order = (const uint32_t *)b_order.p;
+    L.count = (uint32_t)count;
+    L.counter = (uint32_t *)b_counter.p;
+    L.passes = (const DevPass *)b_passes.p;
+    L.phases = (const DevPhase *)b_phases.p;
+    L.ops = (const DevOp *)b_ops.p;
+    L.sops = (const DevOp *)b_sops.p;
+    L.coef = (const double *)b_coef.p;
+    L.scratch = b_scratch.p;
+    L.slot_amps = slot_amps;
+    L.energy = (double *)b_energy.p;
+    L.grad = (double *)b_grad.p;
+    L.grid = grid;
+    prof_begin(DMV_KC_BATCH);
+    DMV_TRY(launch_batch(dtype, L));
+    double bytes = 0.0;
+    for (int i = 0; i < count; i++) {
+        const double sb = std::ldexp((double)amp_bytes(dtype), n[i]);
+        const BatchCircuit &C = circ[i];
+        if (C.small)
+            bytes += 4.0 * sb;  // reads and writes of psi and lambda stay in LDS
+        else
+            bytes += 2.0 * sb * (C.fwd_end - C.fwd_begin) + 4.0 * sb * (C.adj_end - C.adj_begin) + 3.0 * sb;
+    }
+    prof_end(DMV_KC_BATCH, bytes);
+    DMV_TRY(copy_d2h(energy, b_energy.p, (uint64_t)count * sizeof(double)));
+    if (total_params > 0)
+        DMV_TRY(copy_d2h(grad, b_grad.p, (uint64_t)total_params * sizeof(double)));
+    return DMV_OK;
+}
+
+// ---------------------------------------------------------------- autotune --
+static std::string cache_key(int n, int dtype) { return "n" + std::to_string(n) + (dtype == DMV_C64 ? "c64" : "c128"); }
+
+int autotune(dmv_circuit *c, dmv_state *s, int *chosen)
+{
+    DMV_CHECK_ARG(c->n == s->n, "circuit/state size mismatch");
+    DMV_TRY(ensure_device());
+    const int small_cap = s->dtype == DMV_C64 ? kSmallMaxQubitsC64 : kSmallMaxQubitsC128;
+    if (s->n <= small_cap) {
+        if (chosen)
+            *chosen = s->n;
+        return DMV_OK;
+    }
+    std::vector<int> cands = s->dtype == DMV_C64 ? std::vector<int>{11, 12, 13} : std::vector<int>{10, 11, 12};
+    std::vector<int> gens = {8, 16, 32};
+    double best = 1e300;
+    int best_k = 0, best_gen = 24, best_nt = 0;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        set_error("hipEventCreate failed");
+        return DMV_ERR_HIP;
+    }
+    for (int k : cands) {
+        if (k + 6 > s->n)
+            continue;
+        for (int gen : gens)
+            for (int nt = 0; nt < 2; nt++) {
+                tune_set("tile_qubits", k);
+                tune_set("max_gen_per_pass", gen);
+                tune_set("nontemporal", nt);
+                int rc = circuit_forward(c, s, DMV_FORWARD_FUSED);  // builds the plan, warms up
+                if (rc != DMV_OK)
+                    return rc;
+                (void)hipEventRecord(e0, stream());
+                rc = circuit_forward(c, s, DMV_FORWARD_FUSED);
+                (void)hipEventRecord(e1, stream());
+                (void)hipEventSynchronize(e1);
+                if (rc != DMV_OK)
+                    return rc;
+                float ms = 0.f;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                if (ms < best) {
+                    best = ms;
+                    best_k = k;
+                    best_gen = gen;
+                    best_nt = nt;
+                }
+            }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (best_k == 0) {
+        if (chosen)
+            *chosen = 0;
+        return DMV_OK;
+    }
+    tune_set("tile_qubits", best_k);
+    tune_set("max_gen_per_pass", best_gen);
+    tune_set("nontemporal", best_nt);
+    if (const char *path = getenv("DMV_TUNE_CACHE")) {
+        if (FILE *f = fopen(path, "a")) {
+            fprintf(f, "{\"key\": \"%s\", \"tile_qubits\": %d, \"max_gen_per_pass\": %d, \"nontemporal\": %d, \"ms\": %.6f}\n",
+                    cache_key(s->n, s->dtype).c_str(), best_k, best_gen, best_nt, best);
+            fclose(f);
+        }
+    }
+    if (chosen)
+        *chosen = best_k;
+    return DMV_OK;
+}
+
+}  // namespace dmv

@@ -1,0 +1,122 @@
+// Shared definitions of libdamavand-gpu: status codes, error reporting, the
+// device-side plan descriptors consumed by the tile kernels, and small bit
+// helpers used by both host and device code.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "../../include/damavand_gpu.h"
+
+#if defined(__HIPCC__)
+#define DMV_HD __host__ __device__
+#else
+#define DMV_HD
+#endif
+
+namespace dmv {
+
+// ---------------------------------------------------------------- errors --
+void set_error(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
+const char *last_error();
+
+#define DMV_CHECK_ARG(cond, ...)                                                                   \
+    do {                                                                                           \
+        if (!(cond)) {                                                                             \
+            ::dmv::set_error(__VA_ARGS__);                                                         \
+            return DMV_ERR_ARG;                                                                    \
+        }                                                                                          \
+    } while (0)
+
+#define DMV_TRY(expr)                                                                              \
+    do {                                                                                           \
+        int rc_ = (expr);                                                                          \
+        if (rc_ != DMV_OK)                                                                         \
+            return rc_;                                                                            \
+    } while (0)
+
+// ------------------------------------------------------------ constants --
+constexpr int kMaxQubits = 33;      // 2^33 complex128 = 128 GiB, one MI355X
+constexpr int kMaxRegs = 32;        // amplitudes per thread in a tile kernel
+constexpr int kMaxThreadBits = 8;   // 256 threads per tile workgroup
+constexpr int kMaxTileQubits = 13;  // 2^13 * 8 B (c64) or 2^12 * 16 B (c128) = 64 KiB of LDS
+constexpr int kSmallMaxQubitsC128 = 12;  // whole state in one workgroup's LDS (64 KiB)
+constexpr int kSmallMaxQubitsC64 = 13;
+
+inline int amp_bytes(int dtype) { return dtype == DMV_C64 ? 8 : 16; }
+// Low qubits a tile always keeps in lanes so that 8 consecutive lanes read one
+// full 128-byte line: 3 for complex128 (8 x 16 B), 4 for complex64 (16 x 8 B).
+inline int coalesce_bits(int dtype) { return dtype == DMV_C64 ? 4 : 3; }
+
+// ------------------------------------------------------ operation kinds --
+// A tile/gate operation.  GEN: 2x2 matrix on a register-resident target.
+// DIAG: diag(m00, m11) on any bit.  SWAP: X on a register-resident target.
+// IPX/IPY/IPZ: accumulate Im<lambda|P|psi> for P on the target (adjoint).
+enum OpKind : uint32_t {
+    OP_GEN = 0,
+    OP_DIAG = 1,
+    OP_SWAP = 2,
+    OP_IPX = 4,
+    OP_IPY = 5,
+    OP_IPZ = 6,
+};
+
+inline DMV_HD bool op_needs_register(uint32_t kind)
+{
+    return kind == OP_GEN || kind == OP_SWAP || kind == OP_IPX || kind == OP_IPY;
+}
+inline DMV_HD bool op_is_ip(uint32_t kind) { return kind >= OP_IPX && kind <= OP_IPZ; }
+
+// One operation as the kernels read it.  In tile plans `target` and `ctrl`
+// are local tile bits and `rbit` is the target's register bit in its phase;
+// for single-gate and whole-state kernels they are global qubit positions.
+struct alignas(16) DevOp {
+    uint32_t kind;
+    uint32_t target;
+    uint32_t rbit;
+    int32_t slot;      // output slot of an IP op (parameter index), else -1
+    uint64_t ctrl;     // every bit of ctrl must be 1 for the op to act
+    uint64_t pad;
+    double m[8];       // re/im of m00, m01, m10, m11
+};
+static_assert(sizeof(DevOp) == 96, "DevOp layout");
+
+// Register/thread assignment of a tile's k local bits: the local index of
+// register r in thread t is lreg[r] | sum_j bit_j(t) << thr[j].
+struct alignas(16) DevLayout {
+    uint32_t lreg[kMaxRegs];
+    uint32_t thr[kMaxThreadBits];
+};
+
+struct alignas(16) DevPhase {
+    DevLayout lay;
+    uint32_t op_begin, op_end;
+    uint32_t transpose_in;  // 1: go through LDS from the previous layout first
+    uint32_t pad;
+};
+
+// One HBM pass: a tile is the 2^k amplitudes whose index bits outside
+// qpos[0..k) are fixed by the workgroup id.
+struct alignas(16) DevPass {
+    uint64_t greg_ld[kMaxRegs];  // global offset of register r when loading
+    uint64_t greg_st[kMaxRegs];  // ... when storing
+    uint32_t gthr_ld[kMaxThreadBits];  // global bit of thread bit j when loading
+    uint32_t gthr_st[kMaxThreadBits];
+    uint32_t qpos[16];           // global qubit of local bit j, ascending
+    DevLayout lay_ld, lay_st;
+    uint32_t phase_begin, phase_end;
+    uint32_t transpose_st;       // 1: go through LDS to lay_st before storing
+    uint32_t k;
+};
+
+// ---------------------------------------------------------- bit helpers --
+inline DMV_HD uint64_t insert_zero_bit(uint64_t x, uint32_t pos)
+{
+    const uint64_t low = x & ((uint64_t(1) << pos) - 1);
+    return ((x >> pos) << (pos + 1)) | low;
+}
+
+inline int popcount64(uint64_t x) { return __builtin_popcountll(x); }
+
+}  // namespace dmv

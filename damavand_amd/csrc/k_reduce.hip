@@ -1,0 +1,505 @@
+// State utilities, reductions and the sampler.  Every reduction sums in a
+// fixed order (per-thread strided partials, wave64 shuffle tree, per-block
+// partials, then one deterministic workgroup), so results repeat bit for bit.
+#include "device.hpp"
+#include "kernels.hpp"
+
+namespace dmv {
+
+constexpr int RT = 256;          // threads of a reduction block
+constexpr int kBlockBits = 12;   // amplitudes per reduction block: 4096
+
+static uint32_t grid_for(uint64_t items, uint64_t per_block, uint32_t cap = 1u << 20)
+{
+    uint64_t g = (items + per_block - 1) / per_block;
+    if (g < 1)
+        g = 1;
+    return (uint32_t)(g > cap ? cap : g);
+}
+
+// ------------------------------------------------------------- elementwise --
+template <class V>
+__global__ void k_init_basis(V *__restrict__ s, uint64_t n_amp, uint64_t basis)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_amp;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        V v;
+        v.x = i == basis ? 1 : 0;
+        v.y = 0;
+        s[i] = v;
+    }
+}
+
+int launch_init_basis(int dtype, void *state, int n, uint64_t basis)
+{
+    const uint64_t n_amp = uint64_t(1) << n;
+    DMV_CHECK_ARG(basis < n_amp || basis == ~uint64_t(0), "basis index out of range");
+    const uint32_t g = grid_for(n_amp, RT * 8);
+    if (dtype == DMV_C64)
+        hipLaunchKernelGGL(k_init_basis<c64>, dim3(g), dim3(RT), 0, stream(), (c64 *)state, n_amp, basis);
+    else
+        hipLaunchKernelGGL(k_init_basis<c128>, dim3(g), dim3(RT), 0, stream(), (c128 *)state, n_amp, basis);
+    return check_launch("init");
+}
+
+__global__ void k_copy(c128 *__restrict__ d, const c128 *__restrict__ s, uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        d[i] = s[i];
+}
+
+__global__ void k_copy_bytes(uint8_t *__restrict__ d, const uint8_t *__restrict__ s, uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        d[i] = s[i];
+}
+
+int launch_copy(void *dst, const void *src, uint64_t bytes)
+{
+    const uint64_t n16 = bytes / 16;
+    if (n16)
+        hipLaunchKernelGGL(k_copy, dim3(grid_for(n16, RT * 4, 1u << 22)), dim3(RT), 0, stream(), (c128 *)dst,
+                           (const c128 *)src, n16);
+    const uint64_t rest = bytes - n16 * 16;
+    if (rest)
+        hipLaunchKernelGGL(k_copy_bytes, dim3(1), dim3(64), 0, stream(), (uint8_t *)dst + n16 * 16,
+                           (const uint8_t *)src + n16 * 16, rest);
+    return check_launch("copy");
+}
+
+template <class V>
+__global__ void k_split(const V *__restrict__ s, uint64_t begin, uint64_t count, double *re, double *im)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (uint64_t)gridDim.x * blockDim.x) {
+        const V v = s[begin + i];
+        re[i] = v.x;
+        im[i] = v.y;
+    }
+}
+
+template <class V>
+__global__ void k_join(V *__restrict__ s, uint64_t begin, uint64_t count, const double *re, const double *im)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (uint64_t)gridDim.x * blockDim.x) {
+        V v;
+        v.x = re[i];
+        v.y = im[i];
+        s[begin + i] = v;
+    }
+}
+
+template <class V>
+__global__ void k_probs(const V *__restrict__ s, uint64_t begin, uint64_t count, double *out)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (uint64_t)gridDim.x * blockDim.x) {
+        const V v = s[begin + i];
+        out[i] = (double)v.x * v.x + (double)v.y * v.y;
+    }
+}
+
+int launch_split(int dtype, const void *state, uint64_t begin, uint64_t count, double *re, double *im)
+{
+    const uint32_t g = grid_for(count, RT * 4);
+    if (dtype == DMV_C64)
+        hipLaunchKernelGGL(k_split<c64>, dim3(g), dim3(RT), 0, stream(), (const c64 *)state, begin, count, re, im);
+    else
+        hipLaunchKernelGGL(k_split<c128>, dim3(g), dim3(RT), 0, stream(), (const c128 *)state, begin, count, re, im);
+    return check_launch("split");
+}
+
+int launch_join(int dtype, void *state, uint64_t begin, uint64_t count, const double *re, const double *im)
+{
+    const uint32_t g = grid_for(count, RT * 4);
+    if (dtype == DMV_C64)
+        hipLaunchKernelGGL(k_join<c64>, dim3(g), dim3(RT), 0, stream(), (c64 *)state, begin, count, re, im);
+    else
+        hipLaunchKernelGGL(k_join<c128>, dim3(g), dim3(RT), 0, stream(), (c128 *)state, begin, count, re, im);
+    return check_launch("join");
+}
+
+int launch_probs(int dtype, const void *state, uint64_t begin, uint64_t count, double *out)
+{
+    const uint32_t g = grid_for(count, RT * 4);
+    if (dtype == DMV_C64)
+        hipLaunchKernelGGL(k_probs<c64>, dim3(g), dim3(RT), 0, stream(), (const c64 *)state, begin, count, out);
+    else
+        hipLaunchKernelGGL(k_probs<c128>, dim3(g), dim3(RT), 0, stream(), (const c128 *)state, begin, count, out);
+    return check_launch("probabilities");
+}
+
+// --------------------------------------------------------------- reductions --
+template <class V>
+__global__ void __launch_bounds__(RT) k_block_norms(const V *__restrict__ s, uint32_t bb, double *partials)
+{
+    __shared__ double red[RT / 64];
+    const uint64_t base = (uint64_t)blockIdx.x << bb;
+    const uint32_t span = 1u << bb;
+    double acc = 0.0;
+    for (uint32_t i = threadIdx.x; i < span; i += RT)
+        acc += norm2(s[base + i]);
+    acc = block_sum<RT>(acc, red);
+    if (threadIdx.x == 0)
+        partials[blockIdx.x] = acc;
+}
+
+int launch_block_norms(int dtype, const void *state, int n, int bb, double *partials)
+{
+    DMV_CHECK_ARG(bb >= 0 && bb <= n && n - bb <= 31, "block bits out of range");
+    const uint32_t blocks = 1u << (n - bb);
+    if (dtype == DMV_C64)
+        hipLaunchKernelGGL(k_block_norms<c64>, dim3(blocks), dim3(RT), 0, stream(), (const c64 *)state, (uint32_t)bb,
+                           partials);
+    else
+        hipLaunchKernelGGL(k_block_norms<c128>, dim3(blocks), dim3(RT), 0, stream(), (const c128 *)state,
+                           (uint32_t)bb, partials);
+    return check_launch("block norms");
+}
+
+// out[v] = sum_i in[i * stride + v] for v < nv (nv <= 4), one workgroup.
+__global__ void __launch_bounds__(RT) k_sum(const double *__restrict__ in, uint64_t count, uint32_t stride,
+                                            uint32_t nv, double *out)
+{
+    __shared__ double red[RT / 64];
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (uint64_t i = threadIdx.x; i < count; i += RT) {
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+            if ((uint32_t)v < nv)
+                acc[v] += in[i * stride + v];
+    }
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const double s = block_sum<RT>(acc[v], red);
+        if (threadIdx.x == 0 && (uint32_t)v < nv)
+            out[v] = s;
+    }
+}
+
+int launch_sum(const double *in, uint64_t count, double *out)
+{
+    hipLaunchKernelGGL(k_sum, dim3(1), dim3(RT), 0, stream(), in, count, 1u, 1u, out);
+    return check_launch("sum");
+}
+
+// <Z_q>: per block of 2^B amplitudes, the total and, for q < B, the mass
+// with bit q set.  Bits >= B are constant over a block.
+template <class V>
+__global__ void __launch_bounds__(RT) k_expect_z_part(const V *__restrict__ s, uint32_t B, double *partials)
+{
+    __shared__ double red[RT / 64];
+    const uint64_t base = (uint64_t)blockIdx.x << B;
+    const uint32_t span = 1u << B;
+    double tot = 0.0, m[kBlockBits];
+#pragma unroll
+    for (int q = 0; q < kBlockBits; q++)
+        m[q] = 0.0;
+    for (uint32_t i = threadIdx.x; i < span; i += RT) {
+        const double p = norm2(s[base + i]);
+        tot += p;
+#pragma unroll
+        for (int q = 0; q < kBlockBits; q++)
+            if ((i >> q) & 1u)
+                m[q] += p;
+    }
+    double *out = partials + (uint64_t)blockIdx.x * (1 + kBlockBits);
+    const double t = block_sum<RT>(tot, red);
+    if (threadIdx.x == 0)
+        out[0] = t;
+#pragma unroll
+    for (int q = 0; q < kBlockBits; q++) {
+        const double v = block_sum<RT>(m[q], red);
+        if (threadIdx.x == 0)
+            out[1 + q] = v;
+    }
+}
+
+__global__ void __launch_bounds__(RT) k_expect_z_final(const double *__restrict__ partials, uint64_t nblk, uint32_t B,
+                                                       uint32_t n, double *out)
+{
+    __shared__ double red[RT / 64];
+    double tot = 0.0, m[kMaxQubits];
+#pragma unroll
+    for (int q = 0; q < kMaxQubits; q++)
+        m[q] = 0.0;
+    for (uint64_t b = threadIdx.x; b < nblk; b += RT) {
+        const double *pb = partials + b * (1 + kBlockBits);
+        const double t = pb[0];
+        tot += t;
+#pragma unroll
+        for (int q = 0; q < kMaxQubits; q++) {
+            if ((uint32_t)q < B)
+                m[q] += pb[1 + (q < kBlockBits ? q : 0)];
+            else if ((uint32_t)q < n && ((b >> (q - B)) & 1u))
+                m[q] += t;
+        }
+    }
+    const double T = block_sum<RT>(tot, red);
+#pragma unroll
+    for (int q = 0; q < kMaxQubits; q++) {
+        const double v = block_sum<RT>(m[q], red);
+        if (threadIdx.x == 0 && (uint32_t)q < n)
+            out[q] = T - 2.0 * v;
+    }
+}
+
+int launch_expect_z(int dtype, const void *state, int n, double *partials, double *out)
+{
+    const int B = n < kBlockBits ? n : kBlockBits;
+    const uint64_t nblk = uint64_t(1) << (n - B);
+    if (dtype == DMV_C64)
+        hipLaunchKernelGGL(k_expect_z_part<c64>, dim3((uint32_t)nblk), dim3(RT), 0, stream(), (const c64 *)state,
+                           (uint32_t)B, partials);
+    else
+        hipLaunchKernelGGL(k_expect_z_part<c128>, dim3((uint32_t)nblk), dim3(RT), 0, stream(), (const c128 *)state,
+                           (uint32_t)B, partials);
+    DMV_TRY(check_launch("<Z> partials"));
+    hipLaunchKernelGGL(k_expect_z_final, dim3(1), dim3(RT), 0, stream(), partials, nblk, (uint32_t)B, (uint32_t)n, out);
+    return check_launch("<Z> final");
+}
+
+// Z/ZZ observable h(x) = sum_q a_q z_q + sum_t b_t z_{i_t} z_{j_t}, z = 1 - 2 bit.
+struct ObsTerms {
+    const double *a;
+    const int *pi;
+    const int *pj;
+    const double *b;
+    int n_pairs;
+};
+
+constexpr int kMaxPairs = 128;
+
+template <class V, bool WRITE>
+__global__ void __launch_bounds__(RT) k_obs(const V *__restrict__ s, V *__restrict__ lam, uint32_t n,
+                                            uint32_t B, ObsTerms T, double *partials)
+{
+    __shared__ double red[RT / 64];
+    __shared__ double sa[kMaxQubits];
+    __shared__ double sb[kMaxPairs];
+    __shared__ uint64_t smask[kMaxPairs];
+    for (uint32_t q = threadIdx.x; q < n; q += RT)
+        sa[q] = T.a ? T.a[q] : 0.0;
+    for (int t = threadIdx.x; t < T.n_pairs; t += RT) {
+        sb[t] = T.b[t];
+        smask[t] = (uint64_t(1) << T.pi[t]) | (uint64_t(1) << T.pj[t]);
+    }
+    __syncthreads();
+    const uint64_t base = (uint64_t)blockIdx.x << B;
+    const uint32_t span = 1u << B;
+    double acc = 0.0;
+    for (uint32_t i = threadIdx.x; i < span; i += RT) {
+        const uint64_t x = base + i;
+        double h = 0.0;
+        for (uint32_t q = 0; q < n; q++)
+            h += ((x >> q) & 1u) ? -sa[q] : sa[q];
+        for (int t = 0; t < T.n_pairs; t++)
+            h += (__popcll(x & smask[t]) & 1) ? -sb[t] : sb[t];
+        const V v = s[x];
+        acc += h * norm2(v);
+        if constexpr (WRITE) {
+            V w;
+            w.x = v.x * h;
+            w.y = v.y * h;
+            lam[x] = w;
+        }
+    }
+    acc = block_sum<RT>(acc, red);
+    if (threadIdx.x == 0)
+        partials[blockIdx.x] = acc;
+}
+
+template <bool WRITE>
+static int run_obs(int dtype, const void *psi, void *lam, int n, const double *a, int n_pairs, const int *pi,
+                   const int *pj, const double *b, double *partials, double *out)
+{
+    DMV_CHECK_ARG(n_pairs >= 0 && n_pairs <= kMaxPairs, "at most %d ZZ terms", kMaxPairs);
+    const int B = n < kBlockBits ? n : kBlockBits;
+    const uint64_t nblk = uint64_t(1) << (n - B);
+    ObsTerms T{a, pi, pj, b, n_pairs};
+    if (dtype == DMV_C64)
+        hipLaunchKernelGGL((k_obs<c64, WRITE>), dim3((uint32_t)nblk), dim3(RT), 0, stream(), (const c64 *)psi,
+                           (c64 *)lam, (uint32_t)n, (uint32_t)B, T, partials);
+    else
+        hipLaunchKernelGGL((k_obs<c128, WRITE>), dim3((uint32_t)nblk), dim3(RT), 0, stream(), (const c128 *)psi,
+                           (c128 *)lam, (uint32_t)n, (uint32_t)B, T, partials);
+    DMV_TRY(check_launch("<H> partials"));
+    return launch_sum(partials, nblk, out);
+}
+
+int launch_expect_zz(int dtype, const void *state, int n, const double *a, int n_pairs, const int *pi,
+                     const int *pj, const double *b, double *partials, double *out)
+{
+    return run_obs<false>(dtype, state, nullptr, n, a, n_pairs, pi, pj, b, partials, out);
+}
+
+int launch_apply_obs(int dtype, const void *psi, void *lam, int n, const double *a, int n_pairs, const int *pi,
+                     const int *pj, const double *b, double *partials, double *out)
+{
+    return run_obs<true>(dtype, psi, lam, n, a, n_pairs, pi, pj, b, partials, out);
+}
+
+template <class V>
+__global__ void __launch_bounds__(RT) k_inner(const V *__restrict__ a, const V *__restrict__ b, uint32_t B,
+                                              double *partials)
+{
+    __shared__ double red[RT / 64];
+    const uint64_t base = (uint64_t)blockIdx.x << B;
+    const uint32_t span = 1u << B;
+    double re = 0.0, im = 0.0;
+    for (uint32_t i = threadIdx.x; i < span; i += RT) {
+        const V x = a[base + i], y = b[base + i];
+        re += (double)x.x * y.x + (double)x.y * y.y;
+        im += (double)x.x * y.y - (double)x.y * y.x;
+    }
+    re = block_sum<RT>(re, red);
+    im = block_sum<RT>(im, red);
+    if (threadIdx.x == 0) {
+        partials[2 * (uint64_t)blockIdx.x] = re;
+        partials[2 * (uint64_t)blockIdx.x + 1] = im;
+    }
+}
+
+int launch_inner(int dtype, const void *a, const void *b, int n, double *partials, double *out2)
+{
+    const int B = n < kBlockBits ? n : kBlockBits;
+    const uint64_t nblk = uint64_t(1) << (n - B);
+    if (dtype == DMV_C64)
+        hipLaunchKernelGGL(k_inner<c64>, dim3((uint32_t)nblk), dim3(RT), 0, stream(), (const c64 *)a, (const c64 *)b,
+                           (uint32_t)B, partials);
+    else
+        hipLaunchKernelGGL(k_inner<c128>, dim3((uint32_t)nblk), dim3(RT), 0, stream(), (const c128 *)a,
+                           (const c128 *)b, (uint32_t)B, partials);
+    DMV_TRY(check_launch("inner product partials"));
+    hipLaunchKernelGGL(k_sum, dim3(1), dim3(RT), 0, stream(), partials, nblk, 2u, 2u, out2);
+    return check_launch("inner product final");
+}
+
+// One workgroup per slot: out[s] = sum_t partials[s * n_tiles + t].
+__global__ void __launch_bounds__(RT) k_slot_sums(const double *__restrict__ partials, uint64_t n_tiles, double *out)
+{
+    __shared__ double red[RT / 64];
+    const double *p = partials + (uint64_t)blockIdx.x * n_tiles;
+    double acc = 0.0;
+    for (uint64_t t = threadIdx.x; t < n_tiles; t += RT)
+        acc += p[t];
+    acc = block_sum<RT>(acc, red);
+    if (threadIdx.x == 0)
+        out[blockIdx.x] = acc;
+}
+
+int launch_slot_sums(const double *partials, int n_slots, uint64_t n_tiles, double *out)
+{
+    if (n_slots <= 0)
+        return DMV_OK;
+    hipLaunchKernelGGL(k_slot_sums, dim3((uint32_t)n_slots), dim3(RT), 0, stream(), partials, n_tiles, out);
+    return check_launch("slot sums");
+}
+
+// ----------------------------------------------------------------- sampler --
+// Inclusive prefix of the block masses, one workgroup, fixed order.
+__global__ void __launch_bounds__(RT) k_block_scan(const double *__restrict__ in, uint64_t count, double *cum)
+{
+    __shared__ double tsum[RT];
+    const uint64_t seg = (count + RT - 1) / RT;
+    const uint64_t b0 = threadIdx.x * seg;
+    const uint64_t b1 = b0 + seg < count ? b0 + seg : count;
+    double s = 0.0;
+    for (uint64_t i = b0; i < b1; i++)
+        s += in[i];
+    tsum[threadIdx.x] = s;
+    __syncthreads();
+    double off = 0.0;
+    for (uint32_t t = 0; t < threadIdx.x; t++)
+        off += tsum[t];
+    for (uint64_t i = b0; i < b1; i++) {
+        off += in[i];
+        cum[i] = off;
+    }
+}
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// One wave per sample: binary search over the block prefix, then a wave scan
+// of the chosen block, 64 amplitudes at a time.
+template <class V>
+__global__ void __launch_bounds__(RT) k_sample(const V *__restrict__ s, uint32_t bb, uint64_t nblk,
+                                               const double *__restrict__ cum, int64_t k, uint64_t seed,
+                                               const double *__restrict__ u_in, int64_t *__restrict__ out)
+{
+    const int64_t w = ((int64_t)blockIdx.x * RT + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (w >= k)
+        return;
+    double u;
+    if (u_in) {
+        u = u_in[w];
+    } else {
+        const uint64_t z = mix64(seed + (uint64_t)(w + 1) * 0x9E3779B97F4A7C15ull);
+        u = (double)((z >> 11) + 1) * (1.0 / 9007199254740992.0);
+    }
+    const double xsi = u * cum[nblk - 1];
+    uint64_t lo = 0, hi = nblk - 1;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (xsi <= cum[mid])
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    const uint64_t b = lo;
+    double base = b > 0 ? cum[b - 1] : 0.0;
+    const uint32_t span = 1u << bb;
+    const uint64_t first = b << bb;
+    int64_t result = -1, last_nz = -1;
+    for (uint32_t c0 = 0; c0 < span; c0 += 64) {
+        const uint32_t j = c0 + lane;
+        const double p = j < span ? norm2(s[first + j]) : 0.0;
+        double sc = p;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const double t = __shfl_up(sc, off, 64);
+            if (lane >= off)
+                sc += t;
+        }
+        const double c = base + sc;
+        const uint64_t hit = __ballot(p > 0.0 && xsi <= c);
+        const uint64_t nz = __ballot(p > 0.0);
+        if (nz)
+            last_nz = (int64_t)(first + c0 + 63 - __clzll(nz));
+        if (hit) {
+            result = (int64_t)(first + c0 + __ffsll((long long)hit) - 1);
+            break;
+        }
+        base = __shfl(c, 63, 64);
+    }
+    if (result < 0)
+        result = last_nz;
+    if (lane == 0)
+        out[w] = result;
+}
+
+int launch_sample(int dtype, const void *state, int n, int64_t k, uint64_t seed, const double *u,
+                  double *block_cum, double *partials, int64_t *out)
+{
+    const int bb = n < kBlockBits ? n : kBlockBits;
+    const uint64_t nblk = uint64_t(1) << (n - bb);
+    DMV_TRY(launch_block_norms(dtype, state, n, bb, partials));
+    hipLaunchKernelGGL(k_block_scan, dim3(1), dim3(RT), 0, stream(), partials, nblk, block_cum);
+    DMV_TRY(check_launch("block scan"));
+    const uint64_t waves_per_block = RT / 64;
+    const uint64_t blocks = ((uint64_t)k + waves_per_block - 1) / waves_per_block;
+    DMV_CHECK_ARG(blocks <= 0x7FFFFFFFull, "too many samples");
+    if (dtype == DMV_C64)
+        hipLaunchKernelGGL(k_sample<c64>, dim3((uint32_t)blocks), dim3(RT), 0, stream(), (const c64 *)state,
+                           (uint32_t)bb, nblk, block_cum, k, seed, u, out);
+    else
+        hipLaunchKernelGGL(k_sample<c128>, dim3((uint32_t)blocks), dim3(RT), 0, stream(), (const c128 *)state,
+                           (uint32_t)bb, nblk, block_cum, k, seed, u, out);
+    return check_launch("sampler");
+}
+
+}  // namespace dmv

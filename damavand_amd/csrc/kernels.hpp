@@ -1,0 +1,80 @@
+// Launchers of the HIP kernels (host side).  All launch on the library stream
+// and return a DMV_* status; they check their own grid and buffer assumptions.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+
+namespace dmv {
+
+// One fused pass of a tile plan over a state of n qubits.
+int launch_tile(int dtype, int rlog, int tlog, bool nontemporal, void *state, int n,
+                const DevPass *passes, const DevPhase *phases, const DevOp *ops, uint32_t pass_idx);
+// Reverse-sweep tile pass over psi and lambda; IP ops write one partial per
+// (slot, tile) into partials[slot * n_tiles + tile].
+int launch_tile_adjoint(int dtype, int rlog, int tlog, void *psi, void *lam, int n,
+                        const DevPass *passes, const DevPhase *phases, const DevOp *ops,
+                        uint32_t pass_idx, double *partials);
+// Every op of a list applied to a whole state held in one workgroup's LDS.
+int launch_small(int dtype, void *state, int n, const DevOp *ops, uint32_t n_ops);
+// The same for the adjoint reverse sweep (both states in LDS); IP ops write
+// partials[slot].
+int launch_small_adjoint(int dtype, void *psi, void *lam, int n, const DevOp *ops, uint32_t n_ops,
+                         double *partials);
+// Batched persistent launch: one circuit per queue entry.  Circuits with
+// n < k run with psi and lambda in LDS (ops in `sops`, global positions);
+// larger ones run tile passes over per-workgroup scratch states.
+struct BatchCircuit {
+    uint32_t n, small;
+    uint32_t fwd_begin, fwd_end;  // pass range (tile path) or op range (small path)
+    uint32_t adj_begin, adj_end;
+    uint32_t coef;                // a_q at coef[coef + q], b_q at coef[coef + n + q]
+    uint32_t grad;                // gradient of parameter k at grad[grad + k]
+};
+struct BatchLaunch {
+    const BatchCircuit *circ;
+    const uint32_t *order;
+    uint32_t count;
+    uint32_t *counter;
+    const DevPass *passes;
+    const DevPhase *phases;
+    const DevOp *ops;
+    const DevOp *sops;
+    const double *coef;
+    void *scratch;        // grid * 2 * slot_amps amplitudes
+    uint64_t slot_amps;
+    double *energy;
+    double *grad;
+    uint32_t grid;
+};
+int batch_tile_shape(int dtype, int *rlog, int *tlog);
+int launch_batch(int dtype, const BatchLaunch &L);
+
+// One op as its own pass over the amplitudes it changes (global qubits).
+int launch_gate(int dtype, void *state, int n, const DevOp &op, int pairs_per_thread, bool nontemporal);
+
+// --- states, reductions, sampling ---
+int launch_init_basis(int dtype, void *state, int n, uint64_t basis);
+int launch_copy(void *dst, const void *src, uint64_t bytes);
+// Interleaved amplitudes [begin, begin+count) -> split re/im doubles (device).
+int launch_split(int dtype, const void *state, uint64_t begin, uint64_t count, double *re, double *im);
+int launch_join(int dtype, void *state, uint64_t begin, uint64_t count, const double *re, const double *im);
+int launch_probs(int dtype, const void *state, uint64_t begin, uint64_t count, double *out);
+// Partial sums of |a|^2 over blocks of 2^block_bits amplitudes.
+int launch_block_norms(int dtype, const void *state, int n, int block_bits, double *partials);
+// Deterministic single-workgroup sum of `count` doubles into *out (device).
+int launch_sum(const double *in, uint64_t count, double *out);
+int launch_expect_z(int dtype, const void *state, int n, double *partials, double *out);
+int launch_expect_zz(int dtype, const void *state, int n, const double *a, int n_pairs,
+                     const int *pi, const int *pj, const double *b, double *partials, double *out);
+// lambda = h * psi for the Z/ZZ observable, plus <H> into *out.
+int launch_apply_obs(int dtype, const void *psi, void *lam, int n, const double *a, int n_pairs,
+                     const int *pi, const int *pj, const double *b, double *partials, double *out);
+int launch_inner(int dtype, const void *a, const void *b, int n, double *partials, double *out2);
+// Sum each slot's partials: out[s] = sum_t partials[s * n_tiles + t].
+int launch_slot_sums(const double *partials, int n_slots, uint64_t n_tiles, double *out);
+int launch_sample(int dtype, const void *state, int n, int64_t k, uint64_t seed, const double *u,
+                  double *block_cum, double *partials, int64_t *out);
+
+}  // namespace dmv

@@ -1,0 +1,623 @@
+// Circuit lowering and the fused-pass scheduler.  Host C++ only.
+#include "plan.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace dmv {
+
+bool gate_is_parametric(int kind)
+{
+    return kind == DMV_GATE_RX || kind == DMV_GATE_RY || kind == DMV_GATE_RZ;
+}
+
+int gate_num_params(int kind)
+{
+    if (gate_is_parametric(kind))
+        return 1;
+    if (kind == DMV_GATE_U3)
+        return 3;
+    return 0;
+}
+
+// Matrices of src/qubit_backend/gates.rs (H :98-119, CNOT :170-174,
+// X :224-233, Y :289-298, Z :353-362, RX :418-439, RY :490-511, RZ :562-577,
+// S :627-636, T :685-697, Identity :40-49).  U3 and CZ are additions.
+int gate_matrix(int kind, const double *p, double m[8])
+{
+    const double t = p ? p[0] : 0.0;
+    for (int i = 0; i < 8; i++)
+        m[i] = 0.0;
+    switch (kind) {
+    case DMV_GATE_ID:
+        m[0] = 1.0; m[6] = 1.0;
+        return DMV_OK;
+    case DMV_GATE_H:
+        m[0] = 1.0 / std::sqrt(2.0); m[2] = 1.0 / std::sqrt(2.0);
+        m[4] = 1.0 / std::sqrt(2.0); m[6] = -1.0 / std::sqrt(2.0);
+        return DMV_OK;
+    case DMV_GATE_X:
+    case DMV_GATE_CNOT:
+        m[2] = 1.0; m[4] = 1.0;
+        return DMV_OK;
+    case DMV_GATE_Y:
+        m[3] = -1.0; m[5] = 1.0;
+        return DMV_OK;
+    case DMV_GATE_Z:
+    case DMV_GATE_CZ:
+        m[0] = 1.0; m[6] = -1.0;
+        return DMV_OK;
+    case DMV_GATE_RX:
+        m[0] = std::cos(t / 2.0); m[3] = -std::sin(t / 2.0);
+        m[5] = -std::sin(t / 2.0); m[6] = std::cos(t / 2.0);
+        return DMV_OK;
+    case DMV_GATE_RY:
+        m[0] = std::cos(t / 2.0); m[2] = -std::sin(t / 2.0);
+        m[4] = std::sin(t / 2.0); m[6] = std::cos(t / 2.0);
+        return DMV_OK;
+    case DMV_GATE_RZ:
+        m[0] = std::cos(-t / 2.0); m[1] = std::sin(-t / 2.0);
+        m[6] = std::cos(t / 2.0); m[7] = std::sin(t / 2.0);
+        return DMV_OK;
+    case DMV_GATE_S:
+        m[0] = 1.0; m[7] = 1.0;
+        return DMV_OK;
+    case DMV_GATE_T:
+        m[0] = 1.0; m[6] = std::sqrt(2.0) / 2.0; m[7] = std::sqrt(2.0) / 2.0;
+        return DMV_OK;
+    case DMV_GATE_U3: {
+        const double th = p[0], ph = p[1], la = p[2];
+        const double c = std::cos(th / 2.0), s = std::sin(th / 2.0);
+        m[0] = c;
+        m[2] = -std::cos(la) * s; m[3] = -std::sin(la) * s;
+        m[4] = std::cos(ph) * s; m[5] = std::sin(ph) * s;
+        m[6] = std::cos(ph + la) * c; m[7] = std::sin(ph + la) * c;
+        return DMV_OK;
+    }
+    default:
+        set_error("unknown gate kind %d", kind);
+        return DMV_ERR_ARG;
+    }
+}
+
+uint32_t gate_class(int kind)
+{
+    switch (kind) {
+    case DMV_GATE_X:
+    case DMV_GATE_CNOT:
+        return OP_SWAP;
+    case DMV_GATE_Z:
+    case DMV_GATE_CZ:
+    case DMV_GATE_RZ:
+    case DMV_GATE_S:
+    case DMV_GATE_T:
+    case DMV_GATE_ID:
+        return OP_DIAG;
+    default:
+        return OP_GEN;
+    }
+}
+
+static HostOp gate_op(const Gate &g)
+{
+    HostOp op;
+    op.kind = gate_class(g.kind);
+    op.target = g.target;
+    op.ctrl = g.control >= 0 ? (uint64_t(1) << g.control) : 0;
+    gate_matrix(g.kind, g.p, op.m);
+    op.slot = -1;
+    return op;
+}
+
+std::vector<HostOp> forward_ops(const std::vector<Gate> &gates)
+{
+    std::vector<HostOp> out;
+    out.reserve(gates.size());
+    for (const Gate &g : gates)
+        if (!g.observable)
+            out.push_back(gate_op(g));
+    return out;
+}
+
+// Conjugate transpose of a 2x2 stored as m00 m01 m10 m11 (re, im).
+static void dagger(const double m[8], double d[8])
+{
+    d[0] = m[0]; d[1] = -m[1];   // d00 = conj(m00)
+    d[2] = m[4]; d[3] = -m[5];   // d01 = conj(m10)
+    d[4] = m[2]; d[5] = -m[3];   // d10 = conj(m01)
+    d[6] = m[6]; d[7] = -m[7];   // d11 = conj(m11)
+}
+
+std::vector<HostOp> adjoint_ops(const std::vector<Gate> &gates)
+{
+    // parameter index of every gate
+    std::vector<int> pidx(gates.size(), -1);
+    int np = 0;
+    for (size_t i = 0; i < gates.size(); i++)
+        if (gate_is_parametric(gates[i].kind))
+            pidx[i] = np++;
+    std::vector<HostOp> out;
+    out.reserve(2 * gates.size());
+    for (size_t j = gates.size(); j-- > 0;) {
+        const Gate &g = gates[j];
+        if (g.observable)
+            continue;
+        if (pidx[j] >= 0) {
+            HostOp ip;
+            ip.kind = g.kind == DMV_GATE_RX ? OP_IPX : (g.kind == DMV_GATE_RY ? OP_IPY : OP_IPZ);
+            ip.target = g.target;
+            ip.ctrl = 0;
+            std::memset(ip.m, 0, sizeof(ip.m));
+            ip.slot = pidx[j];
+            out.push_back(ip);
+        }
+        HostOp op = gate_op(g);
+        double d[8];
+        dagger(op.m, d);
+        std::memcpy(op.m, d, sizeof(d));
+        out.push_back(op);
+    }
+    return out;
+}
+
+// c = a * b for 2x2 complex matrices.
+static void matmul2(const double a[8], const double b[8], double c[8])
+{
+    for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++) {
+            double re = 0.0, im = 0.0;
+            for (int l = 0; l < 2; l++) {
+                const double ar = a[2 * (2 * i + l)], ai = a[2 * (2 * i + l) + 1];
+                const double br = b[2 * (2 * l + j)], bi = b[2 * (2 * l + j) + 1];
+                re += ar * br - ai * bi;
+                im += ar * bi + ai * br;
+            }
+            c[2 * (2 * i + j)] = re;
+            c[2 * (2 * i + j) + 1] = im;
+        }
+}
+
+static uint32_t merged_class(uint32_t earlier, uint32_t later)
+{
+    if (earlier == OP_DIAG && later == OP_DIAG)
+        return OP_DIAG;
+    if (earlier == OP_SWAP && later == OP_SWAP)
+        return OP_DIAG;  // X * X = I exactly
+    return OP_GEN;
+}
+
+std::vector<HostOp> merge_ops(const std::vector<HostOp> &ops)
+{
+    std::vector<HostOp> out;
+    out.reserve(ops.size());
+    int pending[64];
+    for (int &p : pending)
+        p = -1;
+    for (const HostOp &op : ops) {
+        const bool one_qubit = op.ctrl == 0 && !op_is_ip(op.kind);
+        if (one_qubit) {
+            const int pi = pending[op.target];
+            if (pi >= 0) {
+                HostOp &prev = out[pi];
+                double c[8];
+                matmul2(op.m, prev.m, c);
+                std::memcpy(prev.m, c, sizeof(c));
+                prev.kind = merged_class(prev.kind, op.kind);
+                continue;
+            }
+            out.push_back(op);
+            pending[op.target] = (int)out.size() - 1;
+        } else {
+            out.push_back(op);
+            pending[op.target] = -1;
+            for (int q = 0; q < 64; q++)
+                if ((op.ctrl >> q) & 1)
+                    pending[q] = -1;
+        }
+    }
+    return out;
+}
+
+static uint64_t op_qubits(const HostOp &op) { return (uint64_t(1) << op.target) | op.ctrl; }
+
+// Local index of a global bit mask: bit j of the result is qubit qpos[j].
+static uint64_t to_local(uint64_t global, const uint32_t *qpos, int k)
+{
+    uint64_t l = 0;
+    for (int j = 0; j < k; j++)
+        if ((global >> qpos[j]) & 1)
+            l |= uint64_t(1) << j;
+    return l;
+}
+
+static uint64_t deposit(uint64_t x, const uint32_t *pos, int nbits)
+{
+    uint64_t r = 0;
+    for (int j = 0; j < nbits; j++)
+        if ((x >> j) & 1)
+            r |= uint64_t(1) << pos[j];
+    return r;
+}
+
+// Builds the register/thread layout of a phase whose register bits are `rs`.
+// coalesced: thread bits 0..low_bits-1 are local bits 0..low_bits-1 (they
+// must not be register bits); otherwise thread bits 0..swz_bits-1 get local
+// bits with distinct residues mod swz_bits, which keeps 128-bit LDS accesses
+// of consecutive lanes in distinct banks under the swizzle of the kernels.
+static DevLayout make_layout(uint32_t rs, const PlanParams &pp, bool coalesced)
+{
+    DevLayout L;
+    std::memset(&L, 0, sizeof(L));
+    uint32_t regbits[kMaxRegs];
+    int nr = 0;
+    for (int b = 0; b < pp.k; b++)
+        if ((rs >> b) & 1)
+            regbits[nr++] = (uint32_t)b;
+    for (int r = 0; r < (1 << pp.rlog); r++)
+        L.lreg[r] = (uint32_t)deposit((uint64_t)r, regbits, nr);
+    std::vector<int> tb;
+    for (int b = 0; b < pp.k; b++)
+        if (!((rs >> b) & 1))
+            tb.push_back(b);
+    std::vector<int> order;
+    if (coalesced) {
+        for (int b = 0; b < pp.low_bits; b++)
+            order.push_back(b);
+    } else {
+        for (int c = 0; c < pp.swz_bits && (int)order.size() < (int)tb.size(); c++)
+            for (int b : tb)
+                if (b % pp.swz_bits == c && std::find(order.begin(), order.end(), b) == order.end()) {
+                    order.push_back(b);
+                    break;
+                }
+    }
+    for (int b : tb)
+        if (std::find(order.begin(), order.end(), b) == order.end())
+            order.push_back(b);
+    for (int j = 0; j < pp.tlog; j++)
+        L.thr[j] = (uint32_t)order[j];
+    return L;
+}
+
+static bool layouts_equal(const DevLayout &a, const DevLayout &b, const PlanParams &pp)
+{
+    for (int r = 0; r < (1 << pp.rlog); r++)
+        if (a.lreg[r] != b.lreg[r])
+            return false;
+    for (int j = 0; j < pp.tlog; j++)
+        if (a.thr[j] != b.thr[j])
+            return false;
+    return true;
+}
+
+// Pads a register set to rlog bits, preferring the highest local bits that
+// are not kept in lanes for coalescing.
+static uint32_t pad_regs(uint32_t rs, const PlanParams &pp)
+{
+    for (int b = pp.k - 1; b >= pp.low_bits && popcount64(rs) < pp.rlog; b--)
+        rs |= 1u << b;
+    for (int b = 0; b < pp.k && popcount64(rs) < pp.rlog; b++)
+        rs |= 1u << b;
+    return rs;
+}
+
+static int make_pass(uint64_t Q, const std::vector<HostOp> &taken, const PlanParams &pp, Plan *plan)
+{
+    DevPass P;
+    std::memset(&P, 0, sizeof(P));
+    P.k = (uint32_t)pp.k;
+    int nq = 0;
+    for (int q = 0; q < pp.n; q++)
+        if ((Q >> q) & 1)
+            P.qpos[nq++] = (uint32_t)q;
+    if (nq != pp.k) {
+        set_error("internal: pass has %d qubits, expected %d", nq, pp.k);
+        return DMV_ERR_STATE;
+    }
+    std::vector<HostOp> loc(taken.size());
+    for (size_t i = 0; i < taken.size(); i++) {
+        loc[i] = taken[i];
+        const uint64_t lt = to_local(uint64_t(1) << taken[i].target, P.qpos, pp.k);
+        loc[i].target = __builtin_ctzll(lt);
+        loc[i].ctrl = to_local(taken[i].ctrl, P.qpos, pp.k);
+    }
+
+    // Register phases: greedy, keeping op order wherever ops share a qubit.
+    struct Ph {
+        uint32_t rs;
+        std::vector<int> ops;
+    };
+    std::vector<Ph> phs;
+    std::vector<char> used(loc.size(), 0);
+    size_t left = loc.size();
+    while (left > 0) {
+        Ph ph{0u, {}};
+        uint64_t blocked = 0;
+        for (size_t i = 0; i < loc.size(); i++) {
+            if (used[i])
+                continue;
+            const uint64_t qs = (uint64_t(1) << loc[i].target) | loc[i].ctrl;
+            if (qs & blocked) {
+                blocked |= qs;
+                continue;
+            }
+            if (op_needs_register(loc[i].kind)) {
+                const uint32_t tb = 1u << loc[i].target;
+                if (!(ph.rs & tb)) {
+                    if (popcount64(ph.rs) >= pp.rlog) {
+                        blocked |= qs;
+                        continue;
+                    }
+                    ph.rs |= tb;
+                }
+            }
+            ph.ops.push_back((int)i);
+            used[i] = 1;
+            left--;
+        }
+        if (ph.ops.empty()) {
+            set_error("internal: scheduler made no progress");
+            return DMV_ERR_STATE;
+        }
+        phs.push_back(std::move(ph));
+    }
+    const uint32_t lowmask = (1u << pp.low_bits) - 1;
+    for (Ph &ph : phs)
+        ph.rs = pad_regs(ph.rs, pp);
+
+    // Layouts: load/store layouts keep the low qubits in lanes.
+    const uint32_t default_rs = pad_regs(0u, pp);
+    const bool first_coalesced = (phs.front().rs & lowmask) == 0;
+    const bool last_coalesced = (phs.back().rs & lowmask) == 0;
+    P.lay_ld = make_layout(first_coalesced ? phs.front().rs : default_rs, pp, true);
+    P.phase_begin = (uint32_t)plan->phases.size();
+    const DevLayout *prev = &P.lay_ld;
+    const int pass_index = (int)plan->passes.size();
+    for (size_t p = 0; p < phs.size(); p++) {
+        DevPhase D;
+        std::memset(&D, 0, sizeof(D));
+        const bool coal = (p == 0 && first_coalesced) ||
+                          (p + 1 == phs.size() && last_coalesced && !(p == 0 && !first_coalesced));
+        D.lay = make_layout(phs[p].rs, pp, coal && (phs[p].rs & lowmask) == 0);
+        D.transpose_in = layouts_equal(D.lay, *prev, pp) ? 0u : 1u;
+        D.op_begin = (uint32_t)plan->ops.size();
+        for (int i : phs[p].ops) {
+            const HostOp &h = loc[i];
+            DevOp op;
+            std::memset(&op, 0, sizeof(op));
+            op.kind = h.kind;
+            op.target = (uint32_t)h.target;
+            op.ctrl = h.ctrl;
+            op.slot = h.slot;
+            std::memcpy(op.m, h.m, sizeof(op.m));
+            op.rbit = 0;
+            if (op_needs_register(h.kind)) {
+                int rb = -1;
+                for (int j = 0; j < pp.rlog; j++)
+                    if (D.lay.lreg[1 << j] == (1u << h.target))
+                        rb = j;
+                if (rb < 0) {
+                    set_error("internal: op target not a register bit");
+                    return DMV_ERR_STATE;
+                }
+                op.rbit = (uint32_t)rb;
+            }
+            if (op_is_ip(h.kind) && h.slot + 1 > plan->n_slots)
+                plan->n_slots = h.slot + 1;
+            plan->ops.push_back(op);
+            plan->op_pass.push_back(pass_index);
+        }
+        D.op_end = (uint32_t)plan->ops.size();
+        plan->phases.push_back(D);
+        prev = &plan->phases.back().lay;
+    }
+    P.phase_end = (uint32_t)plan->phases.size();
+    const DevLayout &last = plan->phases.back().lay;
+    if (last_coalesced && layouts_equal(last, make_layout(phs.back().rs, pp, true), pp)) {
+        P.lay_st = last;
+        P.transpose_st = 0;
+    } else {
+        P.lay_st = make_layout(default_rs, pp, true);
+        P.transpose_st = layouts_equal(P.lay_st, last, pp) ? 0u : 1u;
+    }
+    // global offsets of the load/store layouts
+    for (int r = 0; r < (1 << pp.rlog); r++) {
+        P.greg_ld[r] = deposit(P.lay_ld.lreg[r], P.qpos, pp.k);
+        P.greg_st[r] = deposit(P.lay_st.lreg[r], P.qpos, pp.k);
+    }
+    for (int j = 0; j < pp.tlog; j++) {
+        P.gthr_ld[j] = P.qpos[P.lay_ld.thr[j]];
+        P.gthr_st[j] = P.qpos[P.lay_st.thr[j]];
+    }
+    plan->passes.push_back(P);
+    return DMV_OK;
+}
+
+int build_plan(const std::vector<HostOp> &ops, const PlanParams &pp, Plan *out)
+{
+    DMV_CHECK_ARG(pp.k == pp.rlog + pp.tlog, "tile qubits %d != %d + %d", pp.k, pp.rlog, pp.tlog);
+    DMV_CHECK_ARG(pp.n >= pp.k && pp.k <= kMaxTileQubits && pp.rlog >= 1 &&
+                      (1 << pp.rlog) <= kMaxRegs && pp.tlog <= kMaxThreadBits &&
+                      pp.low_bits <= pp.tlog && pp.low_bits + pp.rlog <= pp.k,
+                  "invalid tile shape n=%d k=%d rlog=%d tlog=%d", pp.n, pp.k, pp.rlog, pp.tlog);
+    Plan plan;
+    const uint64_t all = (pp.n >= 64) ? ~uint64_t(0) : ((uint64_t(1) << pp.n) - 1);
+    const uint64_t lowmask = (uint64_t(1) << pp.low_bits) - 1;
+    for (const HostOp &op : ops) {
+        DMV_CHECK_ARG(op.target >= 0 && op.target < pp.n, "target %d out of range", op.target);
+        DMV_CHECK_ARG((op.ctrl & ~all) == 0 && !((op.ctrl >> op.target) & 1), "bad control mask");
+    }
+    std::vector<char> done(ops.size(), 0);
+    size_t first = 0;
+    while (first < ops.size()) {
+        uint64_t Q = lowmask, blocked = 0;
+        int gen = 0;
+        std::vector<HostOp> taken;
+        for (size_t i = first; i < ops.size(); i++) {
+            if (done[i])
+                continue;
+            const uint64_t qs = op_qubits(ops[i]);
+            if (qs & blocked) {
+                blocked |= qs;
+                if ((blocked & all) == all)
+                    break;
+                continue;
+            }
+            const uint64_t nq = Q | qs;
+            const bool budget = ops[i].kind != OP_GEN || gen < pp.max_gen_per_pass;
+            if (popcount64(nq) <= pp.k && budget) {
+                Q = nq;
+                taken.push_back(ops[i]);
+                done[i] = 1;
+                if (ops[i].kind == OP_GEN)
+                    gen++;
+            } else {
+                blocked |= qs;
+                if ((blocked & all) == all)
+                    break;
+            }
+        }
+        while (first < ops.size() && done[first])
+            first++;
+        // fill the tile up to k qubits with the lowest unused qubits
+        for (int q = 0; q < pp.n && popcount64(Q) < pp.k; q++)
+            Q |= uint64_t(1) << q;
+        DMV_TRY(make_pass(Q, taken, pp, &plan));
+    }
+    DMV_TRY(validate_plan(plan, pp));
+    *out = std::move(plan);
+    return DMV_OK;
+}
+
+static int check_layout(const DevLayout &L, const PlanParams &pp, const char *what)
+{
+    uint32_t seen = 0;
+    for (int j = 0; j < pp.tlog; j++) {
+        if (L.thr[j] >= (uint32_t)pp.k || ((seen >> L.thr[j]) & 1)) {
+            set_error("internal: %s thread bit %d invalid", what, j);
+            return DMV_ERR_STATE;
+        }
+        seen |= 1u << L.thr[j];
+    }
+    uint32_t regmask = 0;
+    for (int j = 0; j < pp.rlog; j++)
+        regmask |= L.lreg[1 << j];
+    if ((regmask & seen) != 0 || popcount64(regmask) != pp.rlog || ((regmask | seen) + 1) != (1u << pp.k)) {
+        set_error("internal: %s register bits invalid", what);
+        return DMV_ERR_STATE;
+    }
+    for (int r = 0; r < (1 << pp.rlog); r++) {
+        uint32_t expect = 0;
+        for (int j = 0; j < pp.rlog; j++)
+            if ((r >> j) & 1)
+                expect |= L.lreg[1 << j];
+        if (L.lreg[r] != expect) {
+            set_error("internal: %s register offsets inconsistent", what);
+            return DMV_ERR_STATE;
+        }
+    }
+    return DMV_OK;
+}
+
+int validate_plan(const Plan &plan, const PlanParams &pp)
+{
+    const uint64_t n_amp = uint64_t(1) << pp.n;
+    for (const DevPass &P : plan.passes) {
+        if ((int)P.k != pp.k) {
+            set_error("internal: pass k mismatch");
+            return DMV_ERR_STATE;
+        }
+        for (int j = 0; j < pp.k; j++)
+            if (P.qpos[j] >= (uint32_t)pp.n || (j > 0 && P.qpos[j] <= P.qpos[j - 1])) {
+                set_error("internal: pass qubit list invalid");
+                return DMV_ERR_STATE;
+            }
+        DMV_TRY(check_layout(P.lay_ld, pp, "load layout"));
+        DMV_TRY(check_layout(P.lay_st, pp, "store layout"));
+        for (int j = 0; j < pp.low_bits; j++)
+            if (P.lay_ld.thr[j] != (uint32_t)j || P.lay_st.thr[j] != (uint32_t)j) {
+                set_error("internal: load/store layout not coalesced");
+                return DMV_ERR_STATE;
+            }
+        for (int r = 0; r < (1 << pp.rlog); r++)
+            if (P.greg_ld[r] >= n_amp || P.greg_st[r] >= n_amp) {
+                set_error("internal: register offset out of range");
+                return DMV_ERR_STATE;
+            }
+        for (int j = 0; j < pp.tlog; j++)
+            if (P.gthr_ld[j] >= (uint32_t)pp.n || P.gthr_st[j] >= (uint32_t)pp.n) {
+                set_error("internal: thread bit position out of range");
+                return DMV_ERR_STATE;
+            }
+        if (P.phase_begin >= P.phase_end || P.phase_end > plan.phases.size()) {
+            set_error("internal: phase range invalid");
+            return DMV_ERR_STATE;
+        }
+        for (uint32_t p = P.phase_begin; p < P.phase_end; p++) {
+            const DevPhase &D = plan.phases[p];
+            DMV_TRY(check_layout(D.lay, pp, "phase layout"));
+            if (D.op_begin > D.op_end || D.op_end > plan.ops.size()) {
+                set_error("internal: op range invalid");
+                return DMV_ERR_STATE;
+            }
+            for (uint32_t o = D.op_begin; o < D.op_end; o++) {
+                const DevOp &op = plan.ops[o];
+                if (op.target >= (uint32_t)pp.k || (op.ctrl >> pp.k) != 0 || ((op.ctrl >> op.target) & 1)) {
+                    set_error("internal: op bits out of range");
+                    return DMV_ERR_STATE;
+                }
+                if (op_needs_register(op.kind) &&
+                    (op.rbit >= (uint32_t)pp.rlog || D.lay.lreg[1u << op.rbit] != (1u << op.target))) {
+                    set_error("internal: op register bit inconsistent");
+                    return DMV_ERR_STATE;
+                }
+                if (op.kind != OP_GEN && op.kind != OP_DIAG && op.kind != OP_SWAP && !op_is_ip(op.kind)) {
+                    set_error("internal: op kind invalid");
+                    return DMV_ERR_STATE;
+                }
+            }
+        }
+    }
+    return DMV_OK;
+}
+
+std::vector<HostOp> plan_global_ops(const Plan &plan, std::vector<int> *pass_of_op)
+{
+    std::vector<HostOp> out;
+    for (size_t pi = 0; pi < plan.passes.size(); pi++) {
+        const DevPass &P = plan.passes[pi];
+        for (uint32_t p = P.phase_begin; p < P.phase_end; p++) {
+            const DevPhase &D = plan.phases[p];
+            for (uint32_t o = D.op_begin; o < D.op_end; o++) {
+                const DevOp &d = plan.ops[o];
+                HostOp h;
+                h.kind = d.kind;
+                h.target = (int)P.qpos[d.target];
+                h.ctrl = deposit(d.ctrl, P.qpos, (int)P.k);
+                std::memcpy(h.m, d.m, sizeof(h.m));
+                h.slot = d.slot;
+                out.push_back(h);
+                if (pass_of_op)
+                    pass_of_op->push_back((int)pi);
+            }
+        }
+    }
+    return out;
+}
+
+std::vector<DevOp> global_dev_ops(const std::vector<HostOp> &ops)
+{
+    std::vector<DevOp> out(ops.size());
+    for (size_t i = 0; i < ops.size(); i++) {
+        std::memset(&out[i], 0, sizeof(DevOp));
+        out[i].kind = ops[i].kind;
+        out[i].target = (uint32_t)ops[i].target;
+        out[i].ctrl = ops[i].ctrl;
+        out[i].slot = ops[i].slot;
+        std::memcpy(out[i].m, ops[i].m, sizeof(out[i].m));
+    }
+    return out;
+}
+
+}  // namespace dmv

@@ -1,0 +1,584 @@
+// Device context, profiling, memory, states and circuit execution.
+#include "runtime.hpp"
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+
+#include "device.hpp"
+#include "kernels.hpp"
+
+namespace dmv {
+
+// ------------------------------------------------------------------ errors --
+static thread_local std::string g_error;
+
+void set_error(const char *fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_error = buf;
+}
+
+const char *last_error() { return g_error.c_str(); }
+
+// ----------------------------------------------------------------- context --
+struct ProfRec {
+    hipEvent_t a, b;
+    int kc;
+    double bytes;
+};
+
+struct Context {
+    std::mutex mu;
+    bool init = false;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool prof = false;
+    std::vector<ProfRec> pending;
+    std::vector<hipEvent_t> free_events;
+    hipEvent_t open = nullptr;
+    int64_t launches[DMV_KC_COUNT] = {};
+    double ms[DMV_KC_COUNT] = {};
+    double bytes[DMV_KC_COUNT] = {};
+    std::map<std::string, int64_t> tune;
+};
+
+static Context &ctx()
+{
+    static Context *c = new Context();  // never destroyed: outlives static destructors
+    return *c;
+}
+
+int ensure_device()
+{
+    Context &C = ctx();
+    if (!C.init) {
+        std::lock_guard<std::mutex> lk(C.mu);
+        if (!C.init) {
+            int count = 0;
+            hipError_t e = hipGetDeviceCount(&count);
+            if (e != hipSuccess || count <= 0) {
+                set_error("no GPU visible to HIP (%s)", e == hipSuccess ? "0 devices" : hipGetErrorString(e));
+                return DMV_ERR_NO_DEVICE;
+            }
+            C.device = 0;
+            if ((e = hipSetDevice(C.device)) != hipSuccess) {
+                set_error("hipSetDevice: %s", hipGetErrorString(e));
+                return DMV_ERR_HIP;
+            }
+            // default priority, non-blocking with respect to the null stream
+            if ((e = hipStreamCreateWithFlags(&C.stream, hipStreamNonBlocking)) != hipSuccess) {
+                set_error("hipStreamCreate: %s", hipGetErrorString(e));
+                return DMV_ERR_HIP;
+            }
+            C.init = true;
+        }
+    }
+    hipError_t e = hipSetDevice(C.device);
+    if (e != hipSuccess) {
+        set_error("hipSetDevice: %s", hipGetErrorString(e));
+        return DMV_ERR_HIP;
+    }
+    return DMV_OK;
+}
+
+hipStream_t stream() { return ctx().stream; }
+
+int check_launch(const char *what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error("%s launch failed: %s", what, hipGetErrorString(e));
+        return DMV_ERR_HIP;
+    }
+    return DMV_OK;
+}
+
+#define HIPCALL(call)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess) {                                                                    \
+            set_error("%s: %s", #call, hipGetErrorString(e_));                                     \
+            return DMV_ERR_HIP;                                                                    \
+        }                                                                                          \
+    } while (0)
+
+int synchronize()
+{
+    DMV_TRY(ensure_device());
+    HIPCALL(hipStreamSynchronize(stream()));
+    return DMV_OK;
+}
+
+int alloc(void **p, uint64_t bytes)
+{
+    *p = nullptr;
+    if (bytes == 0)
+        return DMV_OK;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        set_error("device allocation of %llu bytes failed: %s", (unsigned long long)bytes, hipGetErrorString(e));
+        return DMV_ERR_MEMORY;
+    }
+    return DMV_OK;
+}
+
+void release(void *p)
+{
+    if (p) {
+        (void)hipStreamSynchronize(stream());
+        (void)hipFree(p);
+    }
+}
+
+int copy_h2d(void *dst, const void *src, uint64_t bytes)
+{
+    if (bytes == 0)
+        return DMV_OK;
+    HIPCALL(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream()));
+    HIPCALL(hipStreamSynchronize(stream()));
+    return DMV_OK;
+}
+
+int copy_d2h(void *dst, const void *src, uint64_t bytes)
+{
+    if (bytes == 0)
+        return DMV_OK;
+    HIPCALL(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream()));
+    HIPCALL(hipStreamSynchronize(stream()));
+    return DMV_OK;
+}
+
+// --------------------------------------------------------------- profiling --
+static hipEvent_t get_event()
+{
+    Context &C = ctx();
+    if (!C.free_events.empty()) {
+        hipEvent_t e = C.free_events.back();
+        C.free_events.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess)
+        return nullptr;
+    return e;
+}
+
+static void collect()
+{
+    Context &C = ctx();
+    if (C.pending.empty())
+        return;
+    (void)hipStreamSynchronize(C.stream);
+    for (ProfRec &r : C.pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+            C.launches[r.kc]++;
+            C.ms[r.kc] += ms;
+            C.bytes[r.kc] += r.bytes;
+        }
+        C.free_events.push_back(r.a);
+        C.free_events.push_back(r.b);
+    }
+    C.pending.clear();
+}
+
+void prof_begin(int)
+{
+    Context &C = ctx();
+    if (!C.prof)
+        return;
+    C.open = get_event();
+    if (C.open)
+        (void)hipEventRecord(C.open, C.stream);
+}
+
+void prof_end(int kc, double bytes)
+{
+    Context &C = ctx();
+    if (!C.prof || !C.open)
+        return;
+    hipEvent_t b = get_event();
+    if (!b)
+        return;
+    (void)hipEventRecord(b, C.stream);
+    C.pending.push_back({C.open, b, kc, bytes});
+    C.open = nullptr;
+    if (C.pending.size() >= 4096)
+        collect();
+}
+
+int64_t tune_value(const char *name, int64_t dflt)
+{
+    Context &C = ctx();
+    auto it = C.tune.find(name);
+    if (it != C.tune.end())
+        return it->second;
+    const std::string env = std::string("DMV_") + name;
+    std::string up;
+    for (char ch : env)
+        up += (char)toupper((unsigned char)ch);
+    if (const char *v = getenv(up.c_str()))
+        return atoll(v);
+    return dflt;
+}
+
+bool tune_nontemporal() { return tune_value("nontemporal", 0) != 0; }
+
+// ------------------------------------------------------------------- plans --
+PlanParams tile_params(int n, int dtype, bool adjoint)
+{
+    PlanParams pp;
+    pp.n = n;
+    pp.low_bits = coalesce_bits(dtype);
+    pp.swz_bits = dtype == DMV_C64 ? 4 : 3;
+    pp.max_gen_per_pass = (int)tune_value("max_gen_per_pass", 24);
+    int k = (int)tune_value(adjoint ? "adjoint_tile_qubits" : "tile_qubits", 0);
+    int tlog = (int)tune_value("tile_threads_log2", 0);
+    if (k == 0) {
+        if (dtype == DMV_C64)
+            k = adjoint ? 12 : 13;
+        else
+            k = adjoint ? 11 : 12;
+    }
+    if (tlog == 0)
+        tlog = 8;
+    // small registers: fewer threads and smaller tiles keep more workgroups busy
+    if (n < k + 6) {
+        tlog = 6;
+        k = adjoint ? 9 : (dtype == DMV_C64 ? 10 : 9);
+    }
+    if (k > n)
+        k = n;
+    pp.k = k;
+    pp.tlog = tlog;
+    pp.rlog = k - tlog;
+    return pp;
+}
+
+void free_plan(DevicePlan &dp)
+{
+    if (dp.graph) {
+        (void)hipGraphExecDestroy(dp.graph);
+        dp.graph = nullptr;
+    }
+    release(dp.d_passes);
+    release(dp.d_phases);
+    release(dp.d_ops);
+    dp.d_passes = nullptr;
+    dp.d_phases = nullptr;
+    dp.d_ops = nullptr;
+    dp.cap_passes = dp.cap_phases = dp.cap_ops = 0;
+    dp.valid = false;
+}
+
+template <class T> static int upload(T *&dptr, size_t &cap, const std::vector<T> &v)
+{
+    if (v.size() > cap) {
+        release(dptr);
+        dptr = nullptr;
+        const size_t want = v.size() + v.size() / 2 + 4;
+        DMV_TRY(alloc((void **)&dptr, want * sizeof(T)));
+        cap = want;
+    }
+    return copy_h2d(dptr, v.data(), v.size() * sizeof(T));
+}
+
+int prepare_plan(DevicePlan &dp, uint64_t version, int n, int dtype, bool adjoint, const std::vector<HostOp> &ops)
+{
+    const int small_cap = (dtype == DMV_C64 ? kSmallMaxQubitsC64 : kSmallMaxQubitsC128) - (adjoint ? 1 : 0);
+    const bool small = n <= small_cap;
+    PlanParams pp = small ? PlanParams{} : tile_params(n, dtype, adjoint);
+    const bool same_shape = dp.valid && dp.dtype == dtype && dp.small == small &&
+                            (small || (dp.pp.k == pp.k && dp.pp.tlog == pp.tlog && dp.pp.max_gen_per_pass == pp.max_gen_per_pass));
+    if (same_shape && dp.version == version)
+        return DMV_OK;
+    std::vector<HostOp> merged = merge_ops(ops);
+    dp.small = small;
+    dp.dtype = dtype;
+    dp.pp = pp;
+    if (small) {
+        dp.small_ops = global_dev_ops(merged);
+        dp.plan = Plan();
+        for (const DevOp &o : dp.small_ops)
+            if (o.slot + 1 > dp.plan.n_slots)
+                dp.plan.n_slots = o.slot + 1;
+        DMV_TRY(upload(dp.d_ops, dp.cap_ops, dp.small_ops));
+    } else {
+        Plan plan;
+        DMV_TRY(build_plan(merged, pp, &plan));
+        dp.plan = std::move(plan);
+        DMV_TRY(upload(dp.d_passes, dp.cap_passes, dp.plan.passes));
+        DMV_TRY(upload(dp.d_phases, dp.cap_phases, dp.plan.phases));
+        DMV_TRY(upload(dp.d_ops, dp.cap_ops, dp.plan.ops));
+    }
+    dp.version = version;
+    dp.build_id++;
+    dp.valid = true;
+    return DMV_OK;
+}
+
+// ------------------------------------------------------------------ states --
+static uint64_t ws_doubles_for(int n)
+{
+    const int B = n < 12 ? n : 12;
+    const uint64_t nblk = uint64_t(1) << (n - B);
+    return nblk * 13 + 64;
+}
+
+int state_create(int n, int dtype, dmv_state **out)
+{
+    DMV_CHECK_ARG(out != nullptr, "null output pointer");
+    *out = nullptr;
+    DMV_CHECK_ARG(n >= 1 && n <= kMaxQubits, "num_qubits must be in [1, %d], got %d", kMaxQubits, n);
+    DMV_CHECK_ARG(dtype == DMV_C128 || dtype == DMV_C64, "dtype must be DMV_C128 or DMV_C64");
+    DMV_TRY(ensure_device());
+    dmv_state *s = new dmv_state();
+    s->n = n;
+    s->dtype = dtype;
+    s->bytes = (uint64_t(1) << n) * (uint64_t)amp_bytes(dtype);
+    s->ws_doubles = ws_doubles_for(n);
+    const uint64_t nblk = s->ws_doubles;
+    int rc = alloc(&s->amp, s->bytes);
+    if (rc == DMV_OK)
+        rc = alloc((void **)&s->ws, s->ws_doubles * sizeof(double));
+    if (rc == DMV_OK)
+        rc = alloc((void **)&s->cum, nblk * sizeof(double));
+    if (rc == DMV_OK)
+        rc = alloc((void **)&s->scalars, 64 * sizeof(double));
+    if (rc == DMV_OK)
+        rc = state_reset(s, 0, false);
+    if (rc != DMV_OK) {
+        state_destroy(s);
+        return rc;
+    }
+    *out = s;
+    return DMV_OK;
+}
+
+void state_destroy(dmv_state *s)
+{
+    if (!s)
+        return;
+    release(s->amp);
+    release(s->ws);
+    release(s->cum);
+    release(s->scalars);
+    delete s;
+}
+
+int state_reset(dmv_state *s, uint64_t basis, bool zero)
+{
+    DMV_TRY(ensure_device());
+    prof_begin(DMV_KC_REDUCE);
+    DMV_TRY(launch_init_basis(s->dtype, s->amp, s->n, zero ? ~uint64_t(0) : basis));
+    prof_end(DMV_KC_REDUCE, (double)s->bytes);
+    return DMV_OK;
+}
+
+static const uint64_t kChunk = uint64_t(1) << 24;  // amplitudes per staged transfer
+
+static int staging(double **re, double **im, uint64_t count)
+{
+    static double *buf = nullptr;
+    static uint64_t cap = 0;
+    if (count > cap) {
+        release(buf);
+        buf = nullptr;
+        DMV_TRY(alloc((void **)&buf, 2 * count * sizeof(double)));
+        cap = count;
+    }
+    *re = buf;
+    *im = buf + count;
+    return DMV_OK;
+}
+
+int state_get(dmv_state *s, double *re, double *im)
+{
+    DMV_TRY(ensure_device());
+    const uint64_t n_amp = uint64_t(1) << s->n;
+    const uint64_t chunk = n_amp < kChunk ? n_amp : kChunk;
+    double *dre, *dim;
+    DMV_TRY(staging(&dre, &dim, chunk));
+    for (uint64_t b = 0; b < n_amp; b += chunk) {
+        DMV_TRY(launch_split(s->dtype, s->amp, b, chunk, dre, dim));
+        if (re)
+            DMV_TRY(copy_d2h(re + b, dre, chunk * sizeof(double)));
+        if (im)
+            DMV_TRY(copy_d2h(im + b, dim, chunk * sizeof(double)));
+    }
+    return DMV_OK;
+}
+
+int state_set(dmv_state *s, const double *re, const double *im)
+{
+    DMV_TRY(ensure_device());
+    const uint64_t n_amp = uint64_t(1) << s->n;
+    const uint64_t chunk = n_amp < kChunk ? n_amp : kChunk;
+    double *dre, *dim;
+    DMV_TRY(staging(&dre, &dim, chunk));
+    for (uint64_t b = 0; b < n_amp; b += chunk) {
+        DMV_TRY(copy_h2d(dre, re + b, chunk * sizeof(double)));
+        DMV_TRY(copy_h2d(dim, im + b, chunk * sizeof(double)));
+        DMV_TRY(launch_join(s->dtype, s->amp, b, chunk, dre, dim));
+    }
+    return synchronize();
+}
+
+int state_probs(dmv_state *s, double *host)
+{
+    DMV_TRY(ensure_device());
+    const uint64_t n_amp = uint64_t(1) << s->n;
+    const uint64_t chunk = n_amp < kChunk ? n_amp : kChunk;
+    double *dre, *dim;
+    DMV_TRY(staging(&dre, &dim, chunk));
+    for (uint64_t b = 0; b < n_amp; b += chunk) {
+        prof_begin(DMV_KC_REDUCE);
+        DMV_TRY(launch_probs(s->dtype, s->amp, b, chunk, dre));
+        prof_end(DMV_KC_REDUCE, (double)chunk * (amp_bytes(s->dtype) + 8));
+        DMV_TRY(copy_d2h(host + b, dre, chunk * sizeof(double)));
+    }
+    return DMV_OK;
+}
+
+// ---------------------------------------------------------------- circuits --
+static int forward_passes(DevicePlan &dp, dmv_state *s)
+{
+    if (dp.small) {
+        prof_begin(DMV_KC_SMALL);
+        DMV_TRY(launch_small(s->dtype, s->amp, s->n, dp.d_ops, (uint32_t)dp.small_ops.size()));
+        prof_end(DMV_KC_SMALL, 2.0 * (double)s->bytes);
+        return DMV_OK;
+    }
+    const bool nt = tune_nontemporal();
+    for (uint32_t p = 0; p < dp.plan.passes.size(); p++) {
+        prof_begin(DMV_KC_TILE);
+        DMV_TRY(launch_tile(s->dtype, dp.pp.rlog, dp.pp.tlog, nt, s->amp, s->n, dp.d_passes, dp.d_phases, dp.d_ops, p));
+        prof_end(DMV_KC_TILE, 2.0 * (double)s->bytes);
+    }
+    return DMV_OK;
+}
+
+static int forward_graph(DevicePlan &dp, dmv_state *s)
+{
+    Context &C = ctx();
+    if (!dp.graph || dp.graph_state != s->amp || dp.graph_build != dp.build_id) {
+        if (dp.graph) {
+            (void)hipGraphExecDestroy(dp.graph);
+            dp.graph = nullptr;
+        }
+        const bool prof = C.prof;
+        C.prof = false;  // no event records inside the capture
+        HIPCALL(hipStreamBeginCapture(C.stream, hipStreamCaptureModeThreadLocal));
+        int rc = forward_passes(dp, s);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(C.stream, &g);
+        C.prof = prof;
+        if (rc != DMV_OK) {
+            if (g)
+                (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) {
+            set_error("hipStreamEndCapture: %s", hipGetErrorString(e));
+            return DMV_ERR_HIP;
+        }
+        e = hipGraphInstantiate(&dp.graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            dp.graph = nullptr;
+            set_error("hipGraphInstantiate: %s", hipGetErrorString(e));
+            return DMV_ERR_HIP;
+        }
+        dp.graph_state = s->amp;
+        dp.graph_build = dp.build_id;
+    }
+    prof_begin(DMV_KC_TILE);
+    HIPCALL(hipGraphLaunch(dp.graph, C.stream));
+    prof_end(DMV_KC_TILE, 0.0);
+    return DMV_OK;
+}
+
+int circuit_forward(dmv_circuit *c, dmv_state *s, int mode)
+{
+    DMV_CHECK_ARG(c && s, "null circuit or state");
+    DMV_CHECK_ARG(c->n == s->n, "circuit has %d qubits but the state has %d", c->n, s->n);
+    DMV_TRY(ensure_device());
+    if (mode == DMV_FORWARD_PER_GATE) {
+        const int ppt = (int)tune_value("gate_pairs_per_thread", 2);
+        const bool nt = tune_nontemporal();
+        for (const HostOp &h : forward_ops(c->gates)) {
+            DevOp op;
+            std::memset(&op, 0, sizeof(op));
+            op.kind = h.kind;
+            op.target = (uint32_t)h.target;
+            op.ctrl = h.ctrl;
+            op.slot = -1;
+            std::memcpy(op.m, h.m, sizeof(op.m));
+            const uint64_t n_amp = uint64_t(1) << s->n;
+            const int nc = popcount64(h.ctrl);
+            const bool phase_only = h.kind == OP_DIAG && h.m[0] == 1.0 && h.m[1] == 0.0;
+            const double touched = (double)(n_amp >> (nc + (phase_only ? 1 : 0)));
+            prof_begin(DMV_KC_GATE);
+            DMV_TRY(launch_gate(s->dtype, s->amp, s->n, op, ppt, nt));
+            prof_end(DMV_KC_GATE, 2.0 * touched * amp_bytes(s->dtype));
+        }
+        return DMV_OK;
+    }
+    DMV_CHECK_ARG(mode == DMV_FORWARD_FUSED, "unknown forward mode %d", mode);
+    DMV_TRY(prepare_plan(c->fwd, c->version, c->n, s->dtype, false, forward_ops(c->gates)));
+    if (tune_value("graph", 0) && !c->fwd.small)
+        return forward_graph(c->fwd, s);
+    return forward_passes(c->fwd, s);
+}
+
+// -------------------------------------------------------- profiling access --
+int profile_enable(int on)
+{
+    DMV_TRY(ensure_device());
+    collect();
+    ctx().prof = on != 0;
+    return DMV_OK;
+}
+
+int profile_reset()
+{
+    DMV_TRY(ensure_device());
+    collect();
+    Context &C = ctx();
+    for (int i = 0; i < DMV_KC_COUNT; i++) {
+        C.launches[i] = 0;
+        C.ms[i] = 0;
+        C.bytes[i] = 0;
+    }
+    return DMV_OK;
+}
+
+int profile_get(int kc, int64_t *launches, double *ms, double *bytes)
+{
+    DMV_CHECK_ARG(kc >= 0 && kc < DMV_KC_COUNT, "unknown kernel class %d", kc);
+    DMV_TRY(ensure_device());
+    collect();
+    Context &C = ctx();
+    if (launches)
+        *launches = C.launches[kc];
+    if (ms)
+        *ms = C.ms[kc];
+    if (bytes)
+        *bytes = C.bytes[kc];
+    return DMV_OK;
+}
+
+void tune_set(const char *name, int64_t v) { ctx().tune[name] = v; }
+
+}  // namespace dmv

@@ -1,0 +1,85 @@
+// Library runtime: device context, states, circuits and their cached plans.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "common.hpp"
+#include "plan.hpp"
+
+struct dmv_state {
+    int n = 0;
+    int dtype = DMV_C128;
+    void *amp = nullptr;          // 2^n interleaved amplitudes
+    uint64_t bytes = 0;
+    double *ws = nullptr;         // reduction partials
+    uint64_t ws_doubles = 0;
+    double *cum = nullptr;        // sampler block prefix
+    double *scalars = nullptr;    // small device outputs (64 doubles)
+};
+
+namespace dmv {
+
+// A fused plan uploaded to the device.
+struct DevicePlan {
+    bool valid = false;
+    uint64_t version = ~uint64_t(0);
+    uint64_t build_id = 0;        // bumped on every rebuild
+    int dtype = -1;
+    bool small = false;           // whole-state kernel instead of tile passes
+    PlanParams pp{};
+    Plan plan;
+    std::vector<DevOp> small_ops;
+    DevPass *d_passes = nullptr;
+    DevPhase *d_phases = nullptr;
+    DevOp *d_ops = nullptr;
+    size_t cap_passes = 0, cap_phases = 0, cap_ops = 0;
+    // hipGraph of the forward passes on one state buffer
+    hipGraphExec_t graph = nullptr;
+    const void *graph_state = nullptr;
+    uint64_t graph_build = ~uint64_t(0);
+};
+
+}  // namespace dmv
+
+struct dmv_circuit {
+    int n = 0;
+    std::vector<dmv::Gate> gates;
+    uint64_t version = 0;
+    dmv::DevicePlan fwd, adj;
+    double *d_partials = nullptr;   // adjoint per-(slot, tile) partials
+    uint64_t cap_partials = 0;
+    double *d_terms = nullptr;      // observable terms
+    uint64_t cap_terms = 0;
+    double *d_grad = nullptr;
+    uint64_t cap_grad = 0;
+};
+
+namespace dmv {
+
+int ensure_device();              // initialises the context, selects the device
+int synchronize();
+int alloc(void **p, uint64_t bytes);
+void release(void *p);
+int copy_h2d(void *dst, const void *src, uint64_t bytes);
+int copy_d2h(void *dst, const void *src, uint64_t bytes);
+int64_t tune_value(const char *name, int64_t dflt);
+
+// Tile shape for a forward (adjoint = false) or reverse sweep on n qubits.
+PlanParams tile_params(int n, int dtype, bool adjoint);
+// Builds (if stale) and uploads the plan for `ops` into `dp`.
+int prepare_plan(DevicePlan &dp, uint64_t version, int n, int dtype, bool adjoint,
+                 const std::vector<HostOp> &ops);
+void free_plan(DevicePlan &dp);
+
+int state_create(int n, int dtype, dmv_state **out);
+void state_destroy(dmv_state *s);
+int state_reset(dmv_state *s, uint64_t basis, bool zero);
+int state_get(dmv_state *s, double *re, double *im);
+int state_set(dmv_state *s, const double *re, const double *im);
+int state_probs(dmv_state *s, double *host);
+
+int circuit_forward(dmv_circuit *c, dmv_state *s, int mode);
+
+}  // namespace dmv

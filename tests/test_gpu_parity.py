@@ -1,0 +1,373 @@
+"""GPU parity: the HIP path (through the C ABI) against the oracle.
+
+Bars: amplitudes, probabilities and expectations within 1e-12 max-abs in
+complex128; gradients within 1e-10 of the oracle's parameter-shift
+gradient; samples exactly equal to the oracle's for the same uniforms (or
+within 1e-12 of a cumulative boundary), and chi-square consistent for seeded
+draws.  complex64 results are held to 1e-4 and the error is printed.
+"""
+import ctypes
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from damavand_amd import _native as N
+from damavand_amd import workloads as wl
+from damavand_amd.circuit import Circuit, StateVector
+from tests.stat_helpers import chi2_pvalue, marginal_pvalues
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def build_circuit(n, gates, dtype="complex128"):
+    c = Circuit(n, apply_method="gpu", dtype=dtype)
+    for kind, control, target, params in gates:
+        c.add_gate(kind, control, target, params)
+    return c
+
+
+def random_gates(rng, n, count):
+    kinds = [wl.GATE_H, wl.GATE_X, wl.GATE_Y, wl.GATE_Z, wl.GATE_RX, wl.GATE_RY, wl.GATE_RZ, wl.GATE_S, wl.GATE_T,
+             wl.GATE_CNOT, wl.GATE_U3, wl.GATE_CZ]
+    out = []
+    for _ in range(count):
+        k = kinds[rng.integers(len(kinds))]
+        t = int(rng.integers(n))
+        c = -1
+        if k in (wl.GATE_CNOT, wl.GATE_CZ):
+            if n == 1:
+                k = wl.GATE_H
+            else:
+                c = int(rng.integers(n - 1))
+                c += c >= t
+        out.append((k, c, t, tuple(rng.uniform(0, 2 * math.pi, 3))))
+    return out
+
+
+def test_reference_known_answer(native):
+    # circuit.rs:802-837: H(0), H(1), CNOT(0,1) -> all amplitudes 1/2
+    c = Circuit(2, apply_method="gpu")
+    c.add_hadamard_gate(0)
+    c.add_hadamard_gate(1)
+    c.add_cnot_gate(0, 1)
+    c.forward()
+    amp = c.retrieve_amplitudes_on_host()
+    err = np.max(np.abs(amp - 0.5))
+    say("KAT max-abs error", err)
+    assert err <= 1e-15
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 8, 11, 12, 13, 14, 16, 17, 18, 20])
+def test_random_circuit_fused(native, n):
+    say(f"case random fused n={n}")
+    rng = np.random.default_rng(100 + n)
+    gates = random_gates(rng, n, 250)
+    c = build_circuit(n, gates)
+    c.forward()
+    ref = oracle.run_gates(n, gates).amp
+    err = np.max(np.abs(c.retrieve_amplitudes_on_host() - ref))
+    say(f"  n={n} passes={c.plan_stats()[0]} max-abs error {err:.3e}")
+    assert err <= TOL
+
+
+@pytest.mark.parametrize("n", [3, 12, 16])
+def test_random_circuit_per_gate(native, n):
+    say(f"case random per-gate n={n}")
+    rng = np.random.default_rng(200 + n)
+    gates = random_gates(rng, n, 150)
+    c = build_circuit(n, gates)
+    c.forward(per_gate=True)
+    err = np.max(np.abs(c.retrieve_amplitudes_on_host() - oracle.run_gates(n, gates).amp))
+    say(f"  n={n} max-abs error {err:.3e}")
+    assert err <= TOL
+
+
+def test_forward_does_not_reset_state(native):
+    # circuit.rs:341-375 applies the gates on top of the current state
+    gates = [(wl.GATE_RX, -1, 0, (0.3,)), (wl.GATE_CNOT, 0, 1, ())]
+    c = build_circuit(2, gates)
+    c.forward()
+    c.forward()
+    ref = oracle.run_gates(2, gates + gates).amp
+    assert np.max(np.abs(c.retrieve_amplitudes_on_host() - ref)) <= TOL
+
+
+def test_layered_workload_20q(native):
+    """The 990-gate, 20-qubit layered circuit: amplitudes, probabilities, <Z_i>."""
+    say("case layered 20q x 10 layers")
+    n = 20
+    gates = wl.layered_circuit(n, 10, 1001)
+    c = build_circuit(n, gates)
+    c.forward()
+    st = oracle.run_gates(n, gates)
+    e_amp = np.max(np.abs(c.retrieve_amplitudes_on_host() - st.amp))
+    e_p = np.max(np.abs(c.measure() - st.probabilities()))
+    e_z = np.max(np.abs(c.expectation_z() - st.expect_z()))
+    say(f"  amplitudes {e_amp:.3e} probabilities {e_p:.3e} <Z> {e_z:.3e}")
+    assert e_amp <= TOL and e_p <= TOL and e_z <= TOL
+    # per-gate path on the same circuit
+    c.reset_amplitudes()
+    c.forward(per_gate=True)
+    assert np.max(np.abs(c.retrieve_amplitudes_on_host() - st.amp)) <= TOL
+
+
+def test_observables_are_skipped(native):
+    c = Circuit(3, apply_method="gpu")
+    c.add_hadamard_gate(0)
+    c.add_pauli_x_gate(1, True)
+    c.add_pauli_z_gate(0, True)
+    c.add_cnot_gate(0, 2)
+    c.forward()
+    ref = oracle.run_gates(3, [(wl.GATE_H, -1, 0, ()), (wl.GATE_CNOT, 0, 2, ())]).amp
+    assert np.max(np.abs(c.retrieve_amplitudes_on_host() - ref)) <= TOL
+    s = c.sample(2000, seed=5)
+    ev = c.extract_expectation_values(s)
+    assert ev.shape == (2000, 2)
+    assert set(np.unique(s)) <= {0, 5}
+    np.testing.assert_array_equal(ev[:, 0], 1.0)  # qubit 1 never set
+    np.testing.assert_array_equal(ev[:, 1], np.where(s & 1, -1.0, 1.0))
+
+
+def test_compat_layer_call_sequence(native):
+    """The Rust crate's call sequence (circuit.rs:190-195, circuit_gpu.rs:42-58,
+    circuit.rs:573-577, :387-391) against the oracle."""
+    say("case compat layer")
+    L = N.lib()
+    n = 10
+    n_amp = 1 << n
+    assert L.get_number_of_available_gpus() >= 1
+    assert L.get_memory_for_gpu(0) > 1000.0
+    L.init_quantum_state(n_amp, 1, 1)
+    rng = np.random.default_rng(9)
+    gates = random_gates(rng, n, 120)
+    for kind, control, target, params in gates:
+        m = oracle.gate_matrix(wl.GATE_Z if kind == wl.GATE_CZ else kind, params).reshape(4)
+        re = np.ascontiguousarray(m.real)
+        im = np.ascontiguousarray(m.imag)
+        L.apply_one_qubit_gate_gpu_local(N.dptr(re), N.dptr(im), n, n_amp, control, target)
+    probs = np.zeros(n_amp)
+    L.measure_on_gpu(n_amp, N.dptr(probs))
+    re = np.zeros(n_amp)
+    im = np.zeros(n_amp)
+    L.retrieve_amplitudes_on_host(n_amp, N.dptr(re), N.dptr(im))
+    st = oracle.run_gates(n, gates)
+    assert np.max(np.abs(re + 1j * im - st.amp)) <= TOL
+    assert np.max(np.abs(probs - st.probabilities())) <= TOL
+    # the state survives measure (unlike rust_communication.cu:246); measure twice
+    probs2 = np.zeros(n_amp)
+    L.measure_on_gpu(n_amp, N.dptr(probs2))
+    np.testing.assert_array_equal(probs, probs2)
+    # re-init resets to |0>
+    L.init_quantum_state(n_amp, 1, 1)
+    L.retrieve_amplitudes_on_host(n_amp, N.dptr(re), N.dptr(im))
+    assert re[0] == 1.0 and np.count_nonzero(re) == 1 and np.count_nonzero(im) == 0
+
+
+def test_reductions(native):
+    say("case reductions n=15")
+    n = 15
+    rng = np.random.default_rng(3)
+    gates = random_gates(rng, n, 200)
+    c = build_circuit(n, gates)
+    c.forward()
+    st = oracle.run_gates(n, gates)
+    a = rng.normal(size=n)
+    pairs = [(i, i + 1) for i in range(n - 1)] + [(0, 14), (3, 9)]
+    b = rng.normal(size=len(pairs))
+    e1 = abs(c.state.norm_sqr() - 1.0)
+    e2 = np.max(np.abs(c.expectation_z() - st.expect_z()))
+    e3 = abs(c.expectation(a, pairs, b) - st.expect_zz(a, pairs, b))
+    say(f"  norm {e1:.3e} <Z> {e2:.3e} <H> {e3:.3e}")
+    assert e1 <= 1e-12 and e2 <= TOL and e3 <= 1e-11
+
+
+def test_sampler_matches_oracle_on_same_uniforms(native):
+    say("case sampler injected uniforms")
+    n = 13
+    rng = np.random.default_rng(4)
+    gates = random_gates(rng, n, 150)
+    c = build_circuit(n, gates)
+    c.forward()
+    u = wl.sample_uniforms(2024, 20000)
+    got = c.state.sample_uniforms(u)
+    p = oracle.run_gates(n, gates).probabilities()
+    ref = oracle.sample_uniforms(p, u)
+    bad = np.nonzero(got != ref)[0]
+    cum = np.concatenate([[0.0], np.cumsum(p)])
+    for i in bad:  # only allowed within rounding of a cumulative boundary
+        xsi = u[i] * cum[-1]
+        assert np.min(np.abs(cum - xsi)) < 1e-12, (i, got[i], ref[i])
+    say(f"  {len(bad)} of {u.size} draws differ, all at cumulative boundaries")
+    # seeded draws use exactly the published uniform stream
+    np.testing.assert_array_equal(c.state.sample(5000, 77), c.state.sample_uniforms(wl.sample_uniforms(77, 5000)))
+
+
+def test_sampler_chi_square(native):
+    say("case sampler chi-square")
+    n = 10
+    rng = np.random.default_rng(6)
+    c = build_circuit(n, random_gates(rng, n, 100))
+    c.forward()
+    p = c.measure()
+    s = c.sample(200000, seed=11)
+    assert chi2_pvalue(s, p) > 1e-4
+    # 20 qubits: top-bit buckets and per-qubit marginals
+    n = 20
+    c = build_circuit(n, wl.layered_circuit(n, 2, 5))
+    c.forward()
+    p = c.measure()
+    s = c.sample(100000, seed=12)
+    top = 10
+    pb = p.reshape(1 << top, -1).sum(axis=1)
+    assert chi2_pvalue(s >> (n - top), pb) > 1e-4
+    assert min(marginal_pvalues(s, p, n)) > 1e-4 / n
+
+
+def test_sampler_zero_state_raises(native):
+    st = StateVector(4)
+    st.load(np.zeros(16))
+    with pytest.raises(N.DamavandError):
+        st.sample(10, 1)
+
+
+def _param_shift(n, gates, a, pairs, b):
+    """Parameter-shift gradient on the oracle (exact for RX/RY/RZ)."""
+    idx = [i for i, g in enumerate(gates) if g[0] in wl.PARAMETRIC]
+    grad = np.zeros(len(idx))
+    for k, gi in enumerate(idx):
+        vals = []
+        for s in (+0.5 * math.pi, -0.5 * math.pi):
+            g2 = list(gates)
+            kind, c, t, p = g2[gi]
+            g2[gi] = (kind, c, t, (p[0] + s,))
+            vals.append(oracle.run_gates(n, g2).expect_zz(a, pairs, b))
+        grad[k] = 0.5 * (vals[0] - vals[1])
+    return grad
+
+
+@pytest.mark.parametrize("n,layers", [(5, 3), (11, 2), (13, 2), (16, 1)])
+def test_adjoint_vs_parameter_shift(native, n, layers):
+    say(f"case adjoint n={n} layers={layers}")
+    gates, a, b = wl.ansatz_circuit(n, layers, 3000 + n)
+    gates = gates + [(wl.GATE_RX, -1, n // 2, (0.37,))]
+    pairs = wl.chain_pairs(n)
+    c = build_circuit(n, gates)
+    e, g = c.gradient(a, pairs, b)
+    st = oracle.run_gates(n, gates)
+    e_ref = st.expect_zz(a, pairs, b)
+    g_ref = _param_shift(n, gates, a, pairs, b)
+    ee, eg = abs(e - e_ref), np.max(np.abs(g - g_ref))
+    say(f"  energy {ee:.3e} gradient max-abs {eg:.3e} over {g.size} parameters")
+    assert ee <= 1e-11 and eg <= 1e-10
+    # the circuit's own state is untouched by gradient()
+    assert abs(c.state.norm_sqr() - 1.0) < 1e-14
+    assert c.retrieve_amplitudes_on_host()[0] == 1.0
+
+
+def test_adjoint_closed_form(native):
+    # RY(t)|0>, H = Z: E = cos t, dE/dt = -sin t
+    for t in (0.0, 0.4, 2.0, 4.5):
+        c = Circuit(1, apply_method="gpu")
+        c.add_rotation_y_gate(0, t)
+        e, g = c.gradient(np.array([1.0]))
+        assert abs(e - math.cos(t)) < 1e-15 and abs(g[0] + math.sin(t)) < 1e-15
+
+
+def test_fidelity(native):
+    n = 8
+    gates, _, _ = wl.ansatz_circuit(n, 2, 41)
+    c = build_circuit(n, gates)
+    p1 = np.linspace(0, 3, c.num_parameters)
+    p2 = p1 + 0.1
+    f = c.get_fidelity_between_two_states_with_parameters(p1, p2)
+
+    def run(p):
+        g2, k = [], 0
+        for kind, cc, t, pr in gates:
+            if kind in wl.PARAMETRIC:
+                pr = (p[k],)
+                k += 1
+            g2.append((kind, cc, t, pr))
+        return oracle.run_gates(n, g2).amp
+
+    ref = abs(np.vdot(run(p1), run(p2))) ** 2
+    assert abs(f - ref) < 1e-12
+    # circuit_metrics.rs:30: the gate list is cleared afterwards
+    assert c.num_parameters == 0
+
+
+def test_complex64_forward(native):
+    say("case complex64")
+    for n in (9, 14, 19):
+        rng = np.random.default_rng(50 + n)
+        gates = random_gates(rng, n, 200)
+        c = build_circuit(n, gates, dtype="complex64")
+        c.forward()
+        err = np.max(np.abs(c.retrieve_amplitudes_on_host() - oracle.run_gates(n, gates).amp))
+        say(f"  complex64 n={n} max-abs error vs FP64 oracle {err:.3e}")
+        assert err < 1e-4
+
+
+def test_batch_matches_single_circuits(native):
+    say("case batch")
+    circuits = wl.batch_workload(24, 4242, n_range=(3, 14), layer_range=(1, 4))
+    from damavand_amd import batch_forward_adjoint
+    for dtype, tol in (("complex128", 1e-10), ("complex64", 1e-3)):
+        energy, grads = batch_forward_adjoint(circuits, dtype=dtype)
+        worst_e = worst_g = 0.0
+        for i, (n, gates, a, b) in enumerate(circuits):
+            c = build_circuit(n, gates)
+            e, g = c.gradient(a, wl.chain_pairs(n), b)
+            worst_e = max(worst_e, abs(energy[i] - e))
+            worst_g = max(worst_g, float(np.max(np.abs(grads[i] - g))) if g.size else 0.0)
+        say(f"  {dtype}: energy {worst_e:.3e} gradient {worst_g:.3e} vs single-circuit adjoint")
+        assert worst_e <= tol and worst_g <= tol
+    # a few against the oracle's parameter shift
+    for n, gates, a, b in circuits[:3]:
+        energy, grads = batch_forward_adjoint([(n, gates, a, b)])
+        g_ref = _param_shift(n, gates, a, wl.chain_pairs(n), b)
+        assert np.max(np.abs(grads[0] - g_ref)) <= 1e-10
+
+
+def test_hipgraph_replays(native):
+    """Forward replayed from a captured hipGraph equals the eager forward on
+    every replay, with eager work in between."""
+    n = 21
+    gates = wl.layered_circuit(n, 2, 8)
+    ref = build_circuit(n, gates)
+    ref.forward()
+    want = ref.retrieve_amplitudes_on_host()
+    c = build_circuit(n, gates)
+    N.tune_set("graph", 1)
+    try:
+        for rep in range(4):
+            c.reset_amplitudes()
+            c.forward()
+            got = c.retrieve_amplitudes_on_host()
+            assert np.max(np.abs(got - want)) <= TOL, rep
+            c.state.norm_sqr()  # eager work between replays
+    finally:
+        N.tune_set("graph", 0)
+
+
+def test_errors(native):
+    c = Circuit(3, apply_method="gpu")
+    with pytest.raises(N.DamavandError):
+        c.add_hadamard_gate(3)
+    with pytest.raises(N.DamavandError):
+        c.add_cnot_gate(1, 1)
+    st = StateVector(3)
+    with pytest.raises(N.DamavandError):
+        st.apply_matrix(np.eye(2), 5)
+    with pytest.raises(N.DamavandError):
+        st.apply_matrix(np.eye(2), 1, controls=[1])

@@ -1,0 +1,176 @@
+"""CPU-only checks of libdamavand-gpu: it loads, exports every declared
+symbol, and its host logic (gate list, fused-pass scheduler, argument
+checking) is right.  No kernel is launched here."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from damavand_amd import _native as N
+from damavand_amd import workloads as wl
+from damavand_amd.circuit import Circuit, expectation_signs
+from tests.libhelpers import NativeCircuit, header_functions
+
+
+def test_library_exports_every_declared_symbol():
+    lib = N.lib()
+    declared = header_functions()
+    assert len(declared) >= 55
+    missing = [f for f in declared if not hasattr(lib, f)]
+    assert not missing, missing
+    # and the Python binding covers exactly them
+    assert set(declared) == set(N.exported_symbols())
+
+
+def test_reference_compat_symbols_present():
+    # the extern "C" block of circuit.rs:27-51 and circuit_gpu.rs:3-24
+    for name in ["get_number_of_available_gpus", "print_timers", "get_memory_for_gpu", "init_quantum_state",
+                 "measure_on_gpu", "retrieve_amplitudes_on_host", "apply_one_qubit_gate_gpu_local",
+                 "apply_one_qubit_gate_gpu_distributed", "split_amplitudes_between_gpus",
+                 "exchange_amplitudes_between_gpus"]:
+        assert name in header_functions()
+        assert hasattr(N.lib(), name)
+
+
+def test_version_string():
+    assert "gfx950" in N.version()
+
+
+def _oracle_run_plan(n, plan):
+    kind, target, ctrl, mats, _ = plan
+    st = oracle.OracleState(n)
+    for k, t, c, m in zip(kind, target, ctrl, mats):
+        st.apply_matrix_mask(m, int(t), int(c))
+    return st.amp
+
+
+def _random_gates(rng, n, count, kinds=None):
+    kinds = kinds or [wl.GATE_H, wl.GATE_X, wl.GATE_Y, wl.GATE_Z, wl.GATE_RX, wl.GATE_RY, wl.GATE_RZ, wl.GATE_S,
+                      wl.GATE_T, wl.GATE_CNOT, wl.GATE_U3, wl.GATE_CZ]
+    out = []
+    for _ in range(count):
+        k = kinds[rng.integers(len(kinds))]
+        t = int(rng.integers(n))
+        c = -1
+        if k in (wl.GATE_CNOT, wl.GATE_CZ):
+            c = int(rng.integers(n - 1))
+            c += c >= t
+        out.append((k, c, t, tuple(rng.uniform(0, 2 * math.pi, 3))))
+    return out
+
+
+@pytest.mark.parametrize("n,tile,seed", [(13, 0, 1), (14, 9, 2), (16, 12, 3), (16, 10, 4), (18, 11, 5), (15, 13, 6)])
+def test_fused_plan_reproduces_circuit(n, tile, seed):
+    """The scheduler's op sequence (merged matrices, reordered across passes)
+    applied by the oracle equals the oracle run of the original gate list."""
+    rng = np.random.default_rng(seed)
+    gates = _random_gates(rng, n, 300)
+    nc = NativeCircuit(n, gates)
+    dtype = N.C64 if tile == 13 else N.C128
+    plan = nc.plan_ops(dtype=dtype, tile_qubits=tile)
+    ref = oracle.run_gates(n, gates).amp
+    np.testing.assert_allclose(_oracle_run_plan(n, plan), ref, atol=1e-12)
+    # every pass touches at most `tile` qubits, including the coalescing low qubits
+    kind, target, ctrl, _, pas = plan
+    k = tile or 12
+    low = 4 if dtype == N.C64 else 3
+    for p in np.unique(pas):
+        sel = pas == p
+        qs = set(range(low))
+        for t, c in zip(target[sel], ctrl[sel]):
+            qs.add(int(t))
+            qs.update(q for q in range(n) if (int(c) >> q) & 1)
+        assert len(qs) <= k
+
+
+def test_plan_of_layered_workload():
+    n = 22
+    gates = wl.layered_circuit(n, 3, 77)
+    nc = NativeCircuit(n, gates)
+    plan = nc.plan_ops()
+    np.testing.assert_allclose(_oracle_run_plan(n, plan), oracle.run_gates(n, gates).amp, atol=1e-12)
+    passes, phases, ops, hbm = nc.plan_stats()
+    assert passes < len(gates) / 8  # fusion: far fewer HBM passes than gates
+    assert hbm == pytest.approx(2 * 16 * 2 ** n * passes)
+
+
+def test_plan_detects_planted_reordering_mistake():
+    """Swapping two non-commuting ops of the plan must change the result, so
+    the plan comparison above would catch a scheduler that reorders wrongly."""
+    n = 13
+    gates = [(wl.GATE_H, -1, 12, ()), (wl.GATE_CNOT, 12, 0, ()), (wl.GATE_RY, -1, 0, (0.7,))]
+    plan = NativeCircuit(n, gates).plan_ops()
+    kind, target, ctrl, mats, pas = plan
+    order = list(range(len(kind)))
+    order[1], order[2] = order[2], order[1]
+    bad = (kind[order], target[order], ctrl[order], mats[order], pas[order])
+    ref = oracle.run_gates(n, gates).amp
+    assert np.max(np.abs(_oracle_run_plan(n, bad) - ref)) > 1e-3
+
+
+def test_scheduler_merges_single_qubit_runs():
+    n = 14
+    gates = wl.layered_circuit(n, 1, 5)
+    kind, target, ctrl, mats, pas = NativeCircuit(n, gates).plan_ops()
+    # H, RX, RY, RZ on each qubit fold into one op per qubit, CNOTs stay
+    assert len(kind) == n + (n - 1)
+
+
+def test_add_gate_argument_checks():
+    h = NativeCircuit(3, [])
+    p = np.zeros(3)
+    with pytest.raises(N.DamavandError):
+        N.call("dmv_circuit_add_gate", h.h, wl.GATE_H, -1, 3, N.dptr(p), 0, None)   # target out of range
+    with pytest.raises(N.DamavandError):
+        N.call("dmv_circuit_add_gate", h.h, wl.GATE_CNOT, 1, 1, N.dptr(p), 0, None)  # control == target
+    with pytest.raises(N.DamavandError):
+        N.call("dmv_circuit_add_gate", h.h, wl.GATE_H, -1, 0, N.dptr(p), 1, None)    # H is not an observable
+    with pytest.raises(N.DamavandError):
+        N.call("dmv_circuit_add_gate", h.h, 99, -1, 0, N.dptr(p), 0, None)
+    p[0] = float("nan")
+    with pytest.raises(N.DamavandError):
+        N.call("dmv_circuit_add_gate", h.h, wl.GATE_RX, -1, 0, N.dptr(p), 0, None)
+    import ctypes
+    bad = ctypes.c_void_p()
+    with pytest.raises(N.DamavandError):
+        N.call("dmv_circuit_create", 34, ctypes.byref(bad))
+
+
+def test_set_parameters_semantics():
+    # circuit.rs:308-322: zip(parametrised gates, values), truncated to the shorter
+    import ctypes
+    gates = [(wl.GATE_RX, -1, 0, (0.1,)), (wl.GATE_H, -1, 1, ()), (wl.GATE_RZ, -1, 1, (0.2,)),
+             (wl.GATE_RY, -1, 0, (0.3,))]
+    nc = NativeCircuit(2, gates)
+    c = ctypes.c_int(0)
+    N.call("dmv_circuit_num_parameters", nc.h, ctypes.byref(c))
+    assert c.value == 3
+    v = np.array([1.0, 2.0])
+    N.call("dmv_circuit_set_parameters", nc.h, N.dptr(v), 2)
+    out = np.zeros(3)
+    N.call("dmv_circuit_get_parameters", nc.h, N.dptr(out), 3)
+    np.testing.assert_array_equal(out, [1.0, 2.0, 0.3])
+
+
+def test_apply_method_validation():
+    with pytest.raises(ValueError, match="Apply method not recognized"):
+        Circuit(3, apply_method="element_wise")
+    for m in ("brute_force", "shuffle", "multithreading", "distributed_cpu", "distributed_gpu"):
+        with pytest.raises(NotImplementedError):
+            Circuit(3, apply_method=m)
+
+
+def test_no_device_fails_loudly():
+    if N.device_count() > 0:
+        pytest.skip("a GPU is visible")
+    with pytest.raises(N.NoDeviceError):
+        Circuit(3, apply_method="gpu")
+
+
+def test_expectation_signs_follow_reference():
+    # circuit.rs:494-513: -1 if the observable's target bit is set, else +1,
+    # observables in insertion order, Pauli type ignored
+    samples = np.array([0, 1, 2, 3, 5])
+    out = expectation_signs(samples, [0, 2, 0])
+    np.testing.assert_array_equal(out, [[1, 1, 1], [-1, 1, -1], [1, 1, 1], [-1, 1, -1], [-1, -1, -1]])
