@@ -157,6 +157,71 @@ def bench_adjoint(n, layers, shift_params):
     }
 
 
+def bench_batch(count):
+    """4096 ansatz circuits (12-18 qubits, 4-12 layers), forward + adjoint, one launch each dtype."""
+    from damavand_amd import _native as N
+    from damavand_amd import batch_forward_adjoint
+    from damavand_amd import workloads as wl
+    circuits = wl.batch_workload(count, 1004)
+    n_params = sum(wl.count_parameters(g) for _, g, _, _ in circuits)
+    out = {"workload": f"{count} RY/RZ + CNOT-ring circuits, qubits U[12,18], layers U[4,12], Z/ZZ chain H",
+           "circuits": count, "parameters": n_params}
+    res = {}
+    for dtype in ("complex128", "complex64"):
+        batch_forward_adjoint(circuits[:8], dtype=dtype)  # warm-up
+        N.profile_reset()
+        N.profile_enable(True)
+        t0 = time.perf_counter()
+        e, g = batch_forward_adjoint(circuits, dtype=dtype)
+        t = time.perf_counter() - t0
+        N.profile_enable(False)
+        launches, ms, nbytes = N.profile_get(N.KC_BATCH)
+        res[dtype] = (e, g)
+        key = "c128" if dtype == "complex128" else "c64"
+        out[key] = {"call_s": t, "kernel_ms": ms, "circuits_per_s": count / (ms / 1e3),
+                    "gradients_per_s": n_params / (ms / 1e3), "kernel_GBps": nbytes / (ms * 1e6)}
+        log(f"[bench] batch {dtype}: kernel {ms:.1f} ms, call {t:.2f} s, {count / (ms / 1e3):.0f} circuits/s")
+    e128, g128 = res["complex128"]
+    e64, g64 = res["complex64"]
+    out["c64_max_abs_err_energy_vs_c128"] = float(np.max(np.abs(e64 - e128)))
+    out["c64_max_abs_err_grad_vs_c128"] = float(max(np.max(np.abs(a - b)) for a, b in zip(g64, g128)))
+    return out
+
+
+def bench_stress(n, layers, samples):
+    """33-qubit complex128 (128 GiB): U3 on every qubit + 16 CNOT/CZ per layer."""
+    from damavand_amd import _native as N
+    from damavand_amd import workloads as wl
+    gates = wl.stress_circuit(n, layers, 1005)
+    c = build_circuit(n, gates)
+    passes, phases, ops, hbm = c.plan_stats()
+    c.forward()  # plans + warm-up
+    c.reset_amplitudes()
+    sync()
+    N.profile_reset()
+    N.profile_enable(True)
+    t0 = time.perf_counter()
+    c.forward()
+    sync()
+    t = time.perf_counter() - t0
+    N.profile_enable(False)
+    launches, ms, nbytes = N.profile_get(N.KC_TILE)
+    t1 = time.perf_counter()
+    ez = c.expectation_z()
+    t_z = time.perf_counter() - t1
+    t1 = time.perf_counter()
+    s = c.sample(samples, seed=1005)
+    t_s = time.perf_counter() - t1
+    norm = c.state.norm_sqr()
+    log(f"[bench] stress {n}q: {len(gates)} gates, {passes} passes, {t:.2f} s, {hbm / t / 1e9:.0f} GB/s; "
+        f"<Z> {t_z * 1e3:.0f} ms, {samples} samples {t_s * 1e3:.0f} ms, |norm-1| {abs(norm - 1):.2e}")
+    return {"workload": f"{n}-qubit complex128, {layers} layers of U3 on every qubit + 16 CNOT/CZ on seeded pairs",
+            "gates": len(gates), "passes": passes, "forward_s": t, "GBps": hbm / t / 1e9,
+            "tile_kernel_GBps": nbytes / (ms * 1e6) if ms else None, "expect_z_ms": t_z * 1e3,
+            "samples": samples, "sample_ms": t_s * 1e3, "norm_err": abs(norm - 1), "z_first": ez[:4].tolist(),
+            "distinct_samples": int(np.unique(s).size)}
+
+
 def bench_cpu_baseline(n, layers, sample_gates):
     """Oracle (port of the reference's multithreading path) on this host."""
     import oracle
@@ -190,6 +255,9 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-adjoint", action="store_true")
     ap.add_argument("--skip-per-gate", action="store_true")
+    ap.add_argument("--batch", type=int, default=0, help="also run the batched workload with this many circuits")
+    ap.add_argument("--stress", type=int, default=0, help="also run the stress workload at this many qubits")
+    ap.add_argument("--stress-layers", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the state lives on a single MI355X and this build has no "
@@ -206,6 +274,8 @@ def main():
         log(f"[bench] per-gate mode: {per_gate['ms']:.0f} ms, kernels {per_gate['GBps']:.0f} GB/s")
     del circ
     adj = None if args.skip_adjoint else bench_adjoint(args.adjoint_qubits, args.adjoint_layers, args.shift_params)
+    batch = bench_batch(args.batch) if args.batch else None
+    stress = bench_stress(args.stress, args.stress_layers, 100000) if args.stress else None
     cpu = None if args.skip_cpu else bench_cpu_baseline(args.qubits, args.layers, args.cpu_gates)
     if cpu:
         log(f"[bench] cpu baseline {cpu['value']:.2f} GB/s on {cpu['cores']} threads")
@@ -242,6 +312,10 @@ def main():
         "per_gate_mode": per_gate,
         "adjoint": adj,
     }
+    if batch:
+        out["batch"] = batch
+    if stress:
+        out["stress"] = stress
     print(json.dumps(out), flush=True)
 
 

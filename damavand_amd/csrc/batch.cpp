@@ -93,7 +93,7 @@ int batch_forward_adjoint(int count, int dtype, const int *n, const int *gate_of
             pp.tlog = tlog;
             pp.low_bits = coalesce_bits(dtype);
             pp.swz_bits = dtype == DMV_C64 ? 4 : 3;
-            pp.max_gen_per_pass = (int)tune_value("max_gen_per_pass", 24);
+            pp.max_gen_per_pass = (int)tune_value("max_gen_per_pass", 8);
             for (int which = 0; which < 2; which++) {
                 Plan plan;
                 DMV_TRY(build_plan(which == 0 ? f : a, pp, &plan));

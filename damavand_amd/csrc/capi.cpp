@@ -196,7 +196,7 @@ int dmv_apply_matrix(dmv_state *s, const double *m_re, const double *m_im, uint6
     const bool phase_only = op.kind == OP_DIAG && op.m[0] == 1.0 && op.m[1] == 0.0;
     const double touched = (double)(n_amp >> (popcount64(control_mask) + (phase_only ? 1 : 0)));
     prof_begin(DMV_KC_GATE);
-    DMV_TRY(launch_gate(s->dtype, s->amp, s->n, op, (int)tune_value("gate_pairs_per_thread", 2), tune_nontemporal()));
+    DMV_TRY(launch_gate(s->dtype, s->amp, s->n, op, gate_pairs_per_thread(), gate_nontemporal(op.target, s->dtype)));
     prof_end(DMV_KC_GATE, 2.0 * touched * amp_bytes(s->dtype));
     return DMV_OK;
 }

@@ -68,26 +68,52 @@ inline DMV_HD bool op_needs_register(uint32_t kind)
 }
 inline DMV_HD bool op_is_ip(uint32_t kind) { return kind >= OP_IPX && kind <= OP_IPZ; }
 
+constexpr uint32_t kNoReg = 0xFFFFFFFFu;
+
 // One operation as the kernels read it.  In tile plans `target` and `ctrl`
-// are local tile bits and `rbit` is the target's register bit in its phase;
-// for single-gate and whole-state kernels they are global qubit positions.
+// are local tile bits; `rbit` is the target's register bit in its phase
+// (kNoReg when the target is a thread bit, then `tbit` is that thread bit),
+// and the controls are split into register bits (`ctrl_reg`, a mask over the
+// register index, checked per register pair with uniform scalar branches)
+// and thread bits (`ctrl_thr`, a mask over the thread id, checked once per
+// op).  For single-gate and whole-state kernels `target` and `ctrl` are
+// global qubit positions.
 struct alignas(16) DevOp {
     uint32_t kind;
     uint32_t target;
     uint32_t rbit;
+    uint32_t tbit;
     int32_t slot;      // output slot of an IP op (parameter index), else -1
+    uint32_t ctrl_reg;
+    uint32_t ctrl_thr;
+    uint32_t pad0;
     uint64_t ctrl;     // every bit of ctrl must be 1 for the op to act
-    uint64_t pad;
+    uint64_t pad1;
     double m[8];       // re/im of m00, m01, m10, m11
 };
-static_assert(sizeof(DevOp) == 96, "DevOp layout");
+static_assert(sizeof(DevOp) == 112, "DevOp layout");
 
 // Register/thread assignment of a tile's k local bits: the local index of
-// register r in thread t is lreg[r] | sum_j bit_j(t) << thr[j].
+// register r in thread t is lreg[r] | sum_j bit_j(t) << thr[j].  lswz[r] is
+// the LDS byte offset of register r's part of the (linear) swizzle, so the
+// LDS address of (t, r) is swz(thread part) ^ lswz[r].
 struct alignas(16) DevLayout {
     uint32_t lreg[kMaxRegs];
+    uint32_t lswz[kMaxRegs];
     uint32_t thr[kMaxThreadBits];
 };
+
+// LDS address swizzle of a local index: folds the higher G-bit groups into
+// the low G bits.  A local bit p lands on swizzled bit p mod G, so lanes that
+// differ in bits of distinct residues never share a 128-bit bank group.  It is
+// linear over GF(2): swz(a ^ b) = swz(a) ^ swz(b).
+template <int G> inline DMV_HD uint32_t swz_index(uint32_t li)
+{
+    uint32_t f = 0;
+    for (int s = G; s < 16; s += G)
+        f ^= li >> s;
+    return li ^ (f & ((1u << G) - 1));
+}
 
 struct alignas(16) DevPhase {
     DevLayout lay;

@@ -25,6 +25,18 @@ k_tile(typename Cplx<R>::T *__restrict__ state, const DevPass *__restrict__ pass
     tile_body<R, RLOG, TLOG, NT>(state, passes + pass_idx, phases, ops, blockIdx.x, lds);
 }
 
+// Persistent form: gridDim.x workgroups stride over the tiles with the next
+// tile's loads in flight during the current tile's compute.
+template <class R, int RLOG, int TLOG, bool NT>
+__global__ void __launch_bounds__(1 << TLOG)
+k_tile_pipe(typename Cplx<R>::T *__restrict__ state, const DevPass *__restrict__ passes,
+            const DevPhase *__restrict__ phases, const DevOp *__restrict__ ops, uint32_t pass_idx, uint64_t n_tiles)
+{
+    using V = typename Cplx<R>::T;
+    __shared__ V lds[1 << (RLOG + TLOG)];
+    tile_stream<R, RLOG, TLOG, NT>(state, passes + pass_idx, phases, ops, blockIdx.x, gridDim.x, n_tiles, lds);
+}
+
 struct PartialSink {
     double *partials;
     uint64_t n_tiles, tile;
@@ -51,6 +63,19 @@ static int run_tile(bool nt, void *state, int n, const DevPass *passes, const De
 {
     using V = typename Cplx<R>::T;
     const uint64_t tiles = uint64_t(1) << (n - (RLOG + TLOG));
+    const int64_t per_cu = tune_value("tile_wgs_per_cu", 2);
+    const int64_t forced = tune_value("tile_grid", 0);  // explicit persistent grid (tests)
+    const uint64_t grid64 = forced > 0 ? (uint64_t)forced : (uint64_t)device_cus() * (uint64_t)(per_cu > 0 ? per_cu : 1);
+    if (tune_value("tile_pipeline", 0) && per_cu > 0 && tiles >= 2 * grid64) {
+        const uint32_t grid = (uint32_t)grid64;
+        if (nt)
+            hipLaunchKernelGGL((k_tile_pipe<R, RLOG, TLOG, true>), dim3(grid), dim3(1 << TLOG), 0, stream(),
+                               (V *)state, passes, phases, ops, pass_idx, tiles);
+        else
+            hipLaunchKernelGGL((k_tile_pipe<R, RLOG, TLOG, false>), dim3(grid), dim3(1 << TLOG), 0, stream(),
+                               (V *)state, passes, phases, ops, pass_idx, tiles);
+        return check_launch("pipelined tile pass");
+    }
     if (nt)
         hipLaunchKernelGGL((k_tile<R, RLOG, TLOG, true>), dim3((uint32_t)tiles), dim3(1 << TLOG), 0, stream(),
                            (V *)state, passes, phases, ops, pass_idx);

@@ -277,7 +277,28 @@ static DevLayout make_layout(uint32_t rs, const PlanParams &pp, bool coalesced)
             order.push_back(b);
     for (int j = 0; j < pp.tlog; j++)
         L.thr[j] = (uint32_t)order[j];
+    const uint32_t elem = pp.swz_bits == 3 ? 16u : 8u;
+    for (int r = 0; r < (1 << pp.rlog); r++)
+        L.lswz[r] = (pp.swz_bits == 3 ? swz_index<3>(L.lreg[r]) : swz_index<4>(L.lreg[r])) * elem;
     return L;
+}
+
+// Register bit of a local bit in a layout, or kNoReg.
+static uint32_t register_bit(const DevLayout &L, const PlanParams &pp, uint32_t lbit)
+{
+    for (int j = 0; j < pp.rlog; j++)
+        if (L.lreg[1 << j] == (1u << lbit))
+            return (uint32_t)j;
+    return kNoReg;
+}
+
+// Thread bit of a local bit in a layout, or kNoReg.
+static uint32_t thread_bit(const DevLayout &L, const PlanParams &pp, uint32_t lbit)
+{
+    for (int j = 0; j < pp.tlog; j++)
+        if (L.thr[j] == lbit)
+            return (uint32_t)j;
+    return kNoReg;
 }
 
 static bool layouts_equal(const DevLayout &a, const DevLayout &b, const PlanParams &pp)
@@ -391,17 +412,21 @@ static int make_pass(uint64_t Q, const std::vector<HostOp> &taken, const PlanPar
             op.ctrl = h.ctrl;
             op.slot = h.slot;
             std::memcpy(op.m, h.m, sizeof(op.m));
-            op.rbit = 0;
-            if (op_needs_register(h.kind)) {
-                int rb = -1;
-                for (int j = 0; j < pp.rlog; j++)
-                    if (D.lay.lreg[1 << j] == (1u << h.target))
-                        rb = j;
-                if (rb < 0) {
-                    set_error("internal: op target not a register bit");
-                    return DMV_ERR_STATE;
-                }
-                op.rbit = (uint32_t)rb;
+            op.rbit = register_bit(D.lay, pp, (uint32_t)h.target);
+            op.tbit = op.rbit == kNoReg ? thread_bit(D.lay, pp, (uint32_t)h.target) : kNoReg;
+            if (op_needs_register(h.kind) && op.rbit == kNoReg) {
+                set_error("internal: op target not a register bit");
+                return DMV_ERR_STATE;
+            }
+            op.ctrl_reg = op.ctrl_thr = 0;
+            for (int b = 0; b < pp.k; b++) {
+                if (!((h.ctrl >> b) & 1))
+                    continue;
+                const uint32_t rb = register_bit(D.lay, pp, (uint32_t)b);
+                if (rb != kNoReg)
+                    op.ctrl_reg |= 1u << rb;
+                else
+                    op.ctrl_thr |= 1u << thread_bit(D.lay, pp, (uint32_t)b);
             }
             if (op_is_ip(h.kind) && h.slot + 1 > plan->n_slots)
                 plan->n_slots = h.slot + 1;
@@ -507,12 +532,14 @@ static int check_layout(const DevLayout &L, const PlanParams &pp, const char *wh
         set_error("internal: %s register bits invalid", what);
         return DMV_ERR_STATE;
     }
+    const uint32_t elem = pp.swz_bits == 3 ? 16u : 8u;
     for (int r = 0; r < (1 << pp.rlog); r++) {
         uint32_t expect = 0;
         for (int j = 0; j < pp.rlog; j++)
             if ((r >> j) & 1)
                 expect |= L.lreg[1 << j];
-        if (L.lreg[r] != expect) {
+        const uint32_t sw = (pp.swz_bits == 3 ? swz_index<3>(L.lreg[r]) : swz_index<4>(L.lreg[r])) * elem;
+        if (L.lreg[r] != expect || L.lswz[r] != sw || L.lswz[r] >= (elem << pp.k)) {
             set_error("internal: %s register offsets inconsistent", what);
             return DMV_ERR_STATE;
         }
@@ -570,6 +597,29 @@ int validate_plan(const Plan &plan, const PlanParams &pp)
                 if (op_needs_register(op.kind) &&
                     (op.rbit >= (uint32_t)pp.rlog || D.lay.lreg[1u << op.rbit] != (1u << op.target))) {
                     set_error("internal: op register bit inconsistent");
+                    return DMV_ERR_STATE;
+                }
+                const bool reg_ok = op.rbit < (uint32_t)pp.rlog && D.lay.lreg[1u << op.rbit] == (1u << op.target);
+                const bool thr_ok = op.rbit == kNoReg && op.tbit < (uint32_t)pp.tlog && D.lay.thr[op.tbit] == op.target;
+                if (!reg_ok && !thr_ok) {
+                    set_error("internal: op target bit not in the phase layout");
+                    return DMV_ERR_STATE;
+                }
+                uint64_t cm = 0;
+                for (int j = 0; j < 32; j++) {
+                    if ((op.ctrl_reg >> j) & 1) {
+                        if (j >= pp.rlog)
+                            return set_error("internal: register control out of range"), DMV_ERR_STATE;
+                        cm |= D.lay.lreg[1u << j];
+                    }
+                    if ((op.ctrl_thr >> j) & 1) {
+                        if (j >= pp.tlog)
+                            return set_error("internal: thread control out of range"), DMV_ERR_STATE;
+                        cm |= uint64_t(1) << D.lay.thr[j];
+                    }
+                }
+                if (cm != op.ctrl) {
+                    set_error("internal: split controls do not match the control mask");
                     return DMV_ERR_STATE;
                 }
                 if (op.kind != OP_GEN && op.kind != OP_DIAG && op.kind != OP_SWAP && !op_is_ip(op.kind)) {

@@ -92,6 +92,19 @@ int ensure_device()
 
 hipStream_t stream() { return ctx().stream; }
 
+int device_cus()
+{
+    static int cus = 0;
+    if (cus == 0) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, ctx().device) == hipSuccess && prop.multiProcessorCount > 0)
+            cus = prop.multiProcessorCount;
+        else
+            cus = 256;
+    }
+    return cus;
+}
+
 int check_launch(const char *what)
 {
     hipError_t e = hipGetLastError();
@@ -235,6 +248,19 @@ int64_t tune_value(const char *name, int64_t dflt)
 
 bool tune_nontemporal() { return tune_value("nontemporal", 0) != 0; }
 
+// Single-gate kernels: one pair per thread; nontemporal loads/stores unless
+// the target is one of the lowest qubits, where the two streams share lines
+// (measured best on MI355X at 30 qubits).
+int gate_pairs_per_thread() { return (int)tune_value("gate_pairs_per_thread", 1); }
+
+bool gate_nontemporal(uint32_t target, int dtype)
+{
+    const int64_t v = tune_value("gate_nontemporal", -1);
+    if (v >= 0)
+        return v != 0;
+    return target >= (uint32_t)coalesce_bits(dtype);
+}
+
 // ------------------------------------------------------------------- plans --
 PlanParams tile_params(int n, int dtype, bool adjoint)
 {
@@ -242,7 +268,7 @@ PlanParams tile_params(int n, int dtype, bool adjoint)
     pp.n = n;
     pp.low_bits = coalesce_bits(dtype);
     pp.swz_bits = dtype == DMV_C64 ? 4 : 3;
-    pp.max_gen_per_pass = (int)tune_value("max_gen_per_pass", 24);
+    pp.max_gen_per_pass = (int)tune_value("max_gen_per_pass", 8);
     int k = (int)tune_value(adjoint ? "adjoint_tile_qubits" : "tile_qubits", 0);
     int tlog = (int)tune_value("tile_threads_log2", 0);
     if (k == 0) {
@@ -515,9 +541,9 @@ int circuit_forward(dmv_circuit *c, dmv_state *s, int mode)
     DMV_CHECK_ARG(c->n == s->n, "circuit has %d qubits but the state has %d", c->n, s->n);
     DMV_TRY(ensure_device());
     if (mode == DMV_FORWARD_PER_GATE) {
-        const int ppt = (int)tune_value("gate_pairs_per_thread", 2);
-        const bool nt = tune_nontemporal();
+        const int ppt = gate_pairs_per_thread();
         for (const HostOp &h : forward_ops(c->gates)) {
+            const bool nt = gate_nontemporal((uint32_t)h.target, s->dtype);
             DevOp op;
             std::memset(&op, 0, sizeof(op));
             op.kind = h.kind;

@@ -80,6 +80,26 @@ def test_random_circuit_fused(native, n):
     assert err <= TOL
 
 
+@pytest.mark.parametrize("n,grid,nt", [(18, 3, 0), (20, 16, 1), (21, 7, 0)])
+def test_pipelined_tile_passes(native, n, grid, nt):
+    """The persistent, software-pipelined tile kernel (forced onto a small
+    grid so that each workgroup walks several tiles, odd counts included)."""
+    say(f"case pipelined n={n} grid={grid} nontemporal={nt}")
+    rng = np.random.default_rng(300 + n)
+    gates = random_gates(rng, n, 250)
+    c = build_circuit(n, gates)
+    N.tune_set("tile_grid", grid)
+    N.tune_set("nontemporal", nt)
+    try:
+        c.forward()
+    finally:
+        N.tune_set("tile_grid", 0)
+        N.tune_set("nontemporal", 0)
+    err = np.max(np.abs(c.retrieve_amplitudes_on_host() - oracle.run_gates(n, gates).amp))
+    say(f"  max-abs error {err:.3e}")
+    assert err <= TOL
+
+
 @pytest.mark.parametrize("n", [3, 12, 16])
 def test_random_circuit_per_gate(native, n):
     say(f"case random per-gate n={n}")
