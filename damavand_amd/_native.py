@@ -89,6 +89,8 @@ _SIGNATURES = {
     "dmv_circuit_plan_stats": (ctypes.c_int, [_vp, ctypes.c_int, _c_int_p, _c_int_p, _c_int_p, _c_dbl_p]),
     "dmv_circuit_plan_ops": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _c_int_p, _c_int_p, _c_u64_p, _c_dbl_p,
                                             _c_int_p, ctypes.c_int, _c_int_p]),
+    "dmv_circuit_plan_json": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                             ctypes.c_char_p, ctypes.c_uint64, _c_u64_p]),
     "dmv_batch_forward_adjoint": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _c_int_p, _c_int_p, _c_int_p, _c_int_p,
                                                  _c_int_p, _c_dbl_p, _c_int_p, _c_dbl_p, _c_int_p, _c_dbl_p,
                                                  _c_dbl_p]),

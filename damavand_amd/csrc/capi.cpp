@@ -1,7 +1,9 @@
 // The dmv_* C ABI of libdamavand-gpu.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "device.hpp"
@@ -615,6 +617,113 @@ int dmv_circuit_plan_ops(dmv_circuit *c, int dtype, int tile_qubits, int *kind, 
         if (control_mask) control_mask[i] = g[i].ctrl;
         if (m) std::memcpy(m + 8 * i, g[i].m, 8 * sizeof(double));
         if (pass) pass[i] = pass_of[i];
+    }
+    return DMV_OK;
+}
+
+static void json_u32s(std::string &o, const char *key, const uint32_t *a, int n)
+{
+    o += "\"";
+    o += key;
+    o += "\":[";
+    for (int i = 0; i < n; i++) {
+        if (i)
+            o += ",";
+        o += std::to_string(a[i]);
+    }
+    o += "]";
+}
+
+static void json_layout(std::string &o, const char *key, const DevLayout &L, const PlanParams &pp)
+{
+    o += "\"";
+    o += key;
+    o += "\":{";
+    json_u32s(o, "lreg", L.lreg, 1 << pp.rlog);
+    o += ",";
+    json_u32s(o, "lswz", L.lswz, 1 << pp.rlog);
+    o += ",";
+    json_u32s(o, "thr", L.thr, pp.tlog);
+    o += "}";
+}
+
+int dmv_circuit_plan_json(dmv_circuit *c, int dtype, int adjoint, int tile_qubits, int threads_log2,
+                          int max_gen_per_pass, char *buffer, uint64_t capacity, uint64_t *length)
+{
+    CHECK_CIRCUIT(c);
+    PlanParams pp = tile_params(c->n, dtype, adjoint != 0);
+    if (tile_qubits > 0) {
+        pp.k = tile_qubits;
+        pp.tlog = threads_log2 > 0 ? threads_log2 : (tile_qubits >= 10 ? 8 : 6);
+        pp.rlog = pp.k - pp.tlog;
+    }
+    if (max_gen_per_pass > 0)
+        pp.max_gen_per_pass = max_gen_per_pass;
+    std::vector<HostOp> ops = merge_ops(adjoint ? adjoint_ops(c->gates) : forward_ops(c->gates));
+    Plan plan;
+    DMV_TRY(build_plan(ops, pp, &plan));
+    std::string o = "{";
+    o += "\"n\":" + std::to_string(pp.n) + ",\"k\":" + std::to_string(pp.k) + ",\"rlog\":" + std::to_string(pp.rlog) +
+         ",\"tlog\":" + std::to_string(pp.tlog) + ",\"n_slots\":" + std::to_string(plan.n_slots) + ",\"passes\":[";
+    for (size_t i = 0; i < plan.passes.size(); i++) {
+        const DevPass &P = plan.passes[i];
+        if (i)
+            o += ",";
+        o += "{";
+        json_u32s(o, "qpos", P.qpos, pp.k);
+        o += ",";
+        std::vector<uint32_t> gl(1 << pp.rlog), gs(1 << pp.rlog);
+        for (int r = 0; r < (1 << pp.rlog); r++) {
+            gl[r] = (uint32_t)P.greg_ld[r];  // < 2^32 for the sizes this is used at
+            gs[r] = (uint32_t)P.greg_st[r];
+        }
+        json_u32s(o, "greg_ld", gl.data(), 1 << pp.rlog);
+        o += ",";
+        json_u32s(o, "greg_st", gs.data(), 1 << pp.rlog);
+        o += ",";
+        json_u32s(o, "gthr_ld", P.gthr_ld, pp.tlog);
+        o += ",";
+        json_u32s(o, "gthr_st", P.gthr_st, pp.tlog);
+        o += ",";
+        json_layout(o, "lay_ld", P.lay_ld, pp);
+        o += ",";
+        json_layout(o, "lay_st", P.lay_st, pp);
+        o += ",";
+        json_u32s(o, "st_src_lswz", P.st_src_lswz, 1 << pp.rlog);
+        o += ",\"transpose_st\":" + std::to_string(P.transpose_st) + ",\"phases\":[";
+        for (uint32_t p = P.phase_begin; p < P.phase_end; p++) {
+            const DevPhase &D = plan.phases[p];
+            if (p != P.phase_begin)
+                o += ",";
+            o += "{";
+            json_layout(o, "lay", D.lay, pp);
+            o += ",";
+            json_u32s(o, "src_lswz", D.src_lswz, 1 << pp.rlog);
+            o += ",\"transpose_in\":" + std::to_string(D.transpose_in) + ",\"ops\":[";
+            for (uint32_t q = D.op_begin; q < D.op_end; q++) {
+                const DevOp &op = plan.ops[q];
+                if (q != D.op_begin)
+                    o += ",";
+                char buf[512];
+                snprintf(buf, sizeof(buf),
+                         "{\"kind\":%u,\"target\":%u,\"rmask\":%u,\"tbit\":%u,\"slot\":%d,\"en\":%u,\"fl\":%u,"
+                         "\"ctrl_thr\":%u,\"ctrl\":%llu,\"m\":[%.17g,%.17g,%.17g,%.17g,%.17g,%.17g,%.17g,%.17g]}",
+                         op.kind, op.target, op.rmask, op.tbit, op.slot, op.en, op.fl, op.ctrl_thr,
+                         (unsigned long long)op.ctrl, op.m[0], op.m[1], op.m[2], op.m[3], op.m[4], op.m[5], op.m[6],
+                         op.m[7]);
+                o += buf;
+            }
+            o += "]}";
+        }
+        o += "]}";
+    }
+    o += "]}";
+    if (length)
+        *length = o.size() + 1;
+    if (buffer && capacity > 0) {
+        const uint64_t n = std::min<uint64_t>(capacity - 1, o.size());
+        std::memcpy(buffer, o.data(), n);
+        buffer[n] = '\0';
     }
     return DMV_OK;
 }

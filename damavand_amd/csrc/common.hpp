@@ -39,7 +39,7 @@ const char *last_error();
 // ------------------------------------------------------------ constants --
 constexpr int kMaxQubits = 33;      // 2^33 complex128 = 128 GiB, one MI355X
 constexpr int kMaxRegs = 32;        // amplitudes per thread in a tile kernel
-constexpr int kMaxThreadBits = 8;   // 256 threads per tile workgroup
+constexpr int kMaxThreadBits = 9;   // up to 512 threads per tile workgroup
 constexpr int kMaxTileQubits = 13;  // 2^13 * 8 B (c64) or 2^12 * 16 B (c128) = 64 KiB of LDS
 constexpr int kSmallMaxQubitsC128 = 12;  // whole state in one workgroup's LDS (64 KiB)
 constexpr int kSmallMaxQubitsC64 = 13;
@@ -70,23 +70,28 @@ inline DMV_HD bool op_is_ip(uint32_t kind) { return kind >= OP_IPX && kind <= OP
 
 constexpr uint32_t kNoReg = 0xFFFFFFFFu;
 
-// One operation as the kernels read it.  In tile plans `target` and `ctrl`
-// are local tile bits; `rbit` is the target's register bit in its phase
-// (kNoReg when the target is a thread bit, then `tbit` is that thread bit),
-// and the controls are split into register bits (`ctrl_reg`, a mask over the
-// register index, checked per register pair with uniform scalar branches)
-// and thread bits (`ctrl_thr`, a mask over the thread id, checked once per
-// op).  For single-gate and whole-state kernels `target` and `ctrl` are
-// global qubit positions.
+// One operation as the kernels read it.  For single-gate and whole-state
+// kernels `target` and `ctrl` are global qubit positions and nothing else is
+// used.  In tile plans they are local tile bits (kept for checking) and the
+// kernel uses the host-resolved fields:
+//  * rmask: for pair ops (GEN, SWAP, IPX, IPY) the physical register partner
+//    mask (register p pairs with p ^ rmask); for a diagonal or IPZ whose
+//    target is a register bit, the set of physical registers whose target
+//    bit is 1;
+//  * fl: pair ops: the physical registers holding the target-bit-1 member;
+//  * en: the physical registers whose register-bit controls hold;
+//  * tbit: the thread bit of a diagonal/IPZ target that is not a register
+//    bit (kNoReg otherwise);
+//  * ctrl_thr: controls that are thread bits (mask over the thread id).
 struct alignas(16) DevOp {
     uint32_t kind;
     uint32_t target;
-    uint32_t rbit;
+    uint32_t rmask;
     uint32_t tbit;
     int32_t slot;      // output slot of an IP op (parameter index), else -1
-    uint32_t ctrl_reg;
+    uint32_t en;
+    uint32_t fl;
     uint32_t ctrl_thr;
-    uint32_t pad0;
     uint64_t ctrl;     // every bit of ctrl must be 1 for the op to act
     uint64_t pad1;
     double m[8];       // re/im of m00, m01, m10, m11
@@ -117,6 +122,7 @@ template <int G> inline DMV_HD uint32_t swz_index(uint32_t li)
 
 struct alignas(16) DevPhase {
     DevLayout lay;
+    uint32_t src_lswz[kMaxRegs];  // LDS byte offsets of the incoming transpose's writes
     uint32_t op_begin, op_end;
     uint32_t transpose_in;  // 1: go through LDS from the previous layout first
     uint32_t pad;
@@ -131,6 +137,7 @@ struct alignas(16) DevPass {
     uint32_t gthr_st[kMaxThreadBits];
     uint32_t qpos[16];           // global qubit of local bit j, ascending
     DevLayout lay_ld, lay_st;
+    uint32_t st_src_lswz[kMaxRegs];  // LDS byte offsets of the store transpose's writes
     uint32_t phase_begin, phase_end;
     uint32_t transpose_st;       // 1: go through LDS to lay_st before storing
     uint32_t k;

@@ -90,6 +90,8 @@ int launch_tile(int dtype, int rlog, int tlog, bool nt, void *state, int n, cons
 {
     DMV_CHECK_ARG(n >= rlog + tlog && n - (rlog + tlog) <= 31, "tile shape does not fit %d qubits", n);
     if (dtype == DMV_C128) {
+        if (tlog == 9 && rlog == 4) return run_tile<double, 4, 9>(nt, state, n, passes, phases, ops, pass_idx);
+        if (tlog == 9 && rlog == 3) return run_tile<double, 3, 9>(nt, state, n, passes, phases, ops, pass_idx);
         if (tlog == 8 && rlog == 4) return run_tile<double, 4, 8>(nt, state, n, passes, phases, ops, pass_idx);
         if (tlog == 8 && rlog == 3) return run_tile<double, 3, 8>(nt, state, n, passes, phases, ops, pass_idx);
         if (tlog == 8 && rlog == 2) return run_tile<double, 2, 8>(nt, state, n, passes, phases, ops, pass_idx);

@@ -393,63 +393,121 @@ static int make_pass(uint64_t Q, const std::vector<HostOp> &taken, const PlanPar
     const bool last_coalesced = (phs.back().rs & lowmask) == 0;
     P.lay_ld = make_layout(first_coalesced ? phs.front().rs : default_rs, pp, true);
     P.phase_begin = (uint32_t)plan->phases.size();
-    const DevLayout *prev = &P.lay_ld;
     const int pass_index = (int)plan->passes.size();
+    const int NR = 1 << pp.rlog;
+    const uint32_t all_regs = NR >= 32 ? 0xFFFFFFFFu : ((1u << NR) - 1);
+
+    // Register renaming: physical register p holds logical register tab[p]
+    // (an affine map over GF(2)).  X gates and CNOTs between register bits
+    // only update the map; every later op addresses physical registers
+    // through it, and the next transpose or store writes register p to the
+    // place of logical register tab[p].
+    uint32_t tab[kMaxRegs], inv[kMaxRegs];
+    auto reset_tab = [&]() {
+        for (int q = 0; q < NR; q++)
+            tab[q] = (uint32_t)q;
+    };
+    auto invert = [&]() {
+        for (int q = 0; q < NR; q++)
+            inv[tab[q]] = (uint32_t)q;
+    };
+    reset_tab();
+    DevLayout prev = P.lay_ld;
     for (size_t p = 0; p < phs.size(); p++) {
         DevPhase D;
         std::memset(&D, 0, sizeof(D));
         const bool coal = (p == 0 && first_coalesced) ||
                           (p + 1 == phs.size() && last_coalesced && !(p == 0 && !first_coalesced));
         D.lay = make_layout(phs[p].rs, pp, coal && (phs[p].rs & lowmask) == 0);
-        D.transpose_in = layouts_equal(D.lay, *prev, pp) ? 0u : 1u;
+        D.transpose_in = layouts_equal(D.lay, prev, pp) ? 0u : 1u;
+        if (D.transpose_in) {
+            for (int q = 0; q < NR; q++)
+                D.src_lswz[q] = prev.lswz[tab[q]];
+            reset_tab();
+        }
         D.op_begin = (uint32_t)plan->ops.size();
         for (int i : phs[p].ops) {
             const HostOp &h = loc[i];
+            {
+                HostOp g = h;
+                g.target = (int)P.qpos[h.target];
+                g.ctrl = deposit(h.ctrl, P.qpos, pp.k);
+                plan->logical.push_back(g);
+                plan->logical_pass.push_back(pass_index);
+            }
+            const uint32_t J = register_bit(D.lay, pp, (uint32_t)h.target);
+            uint32_t creg = 0, cthr = 0;
+            for (int b = 0; b < pp.k; b++) {
+                if (!((h.ctrl >> b) & 1))
+                    continue;
+                const uint32_t rb = register_bit(D.lay, pp, (uint32_t)b);
+                if (rb != kNoReg)
+                    creg |= 1u << rb;
+                else
+                    cthr |= 1u << thread_bit(D.lay, pp, (uint32_t)b);
+            }
+            if (op_needs_register(h.kind) && J == kNoReg) {
+                set_error("internal: op target not a register bit");
+                return DMV_ERR_STATE;
+            }
+            // relabel-only X / CNOT: the map stays affine with at most one register control
+            if (h.kind == OP_SWAP && cthr == 0 && popcount64(creg) <= 1 && tune_value("register_renaming", 1)) {
+                for (int q = 0; q < NR; q++)
+                    if ((tab[q] & creg) == creg)
+                        tab[q] ^= 1u << J;
+                continue;
+            }
+            invert();
             DevOp op;
             std::memset(&op, 0, sizeof(op));
             op.kind = h.kind;
             op.target = (uint32_t)h.target;
             op.ctrl = h.ctrl;
             op.slot = h.slot;
+            op.ctrl_thr = cthr;
             std::memcpy(op.m, h.m, sizeof(op.m));
-            op.rbit = register_bit(D.lay, pp, (uint32_t)h.target);
-            op.tbit = op.rbit == kNoReg ? thread_bit(D.lay, pp, (uint32_t)h.target) : kNoReg;
-            if (op_needs_register(h.kind) && op.rbit == kNoReg) {
-                set_error("internal: op target not a register bit");
-                return DMV_ERR_STATE;
-            }
-            op.ctrl_reg = op.ctrl_thr = 0;
-            for (int b = 0; b < pp.k; b++) {
-                if (!((h.ctrl >> b) & 1))
-                    continue;
-                const uint32_t rb = register_bit(D.lay, pp, (uint32_t)b);
-                if (rb != kNoReg)
-                    op.ctrl_reg |= 1u << rb;
-                else
-                    op.ctrl_thr |= 1u << thread_bit(D.lay, pp, (uint32_t)b);
+            op.tbit = J == kNoReg ? thread_bit(D.lay, pp, (uint32_t)h.target) : kNoReg;
+            op.en = 0;
+            for (int q = 0; q < NR; q++)
+                if ((tab[q] & creg) == creg)
+                    op.en |= 1u << q;
+            if (J != kNoReg) {
+                op.fl = 0;
+                for (int q = 0; q < NR; q++)
+                    if ((tab[q] >> J) & 1)
+                        op.fl |= 1u << q;
+                // partner of physical q: the physical register of logical tab[q] ^ e_J
+                op.rmask = inv[tab[0] ^ (1u << J)] ^ inv[tab[0]];
+                if (!op_needs_register(h.kind))
+                    op.rmask = op.fl;  // register-target diagonal: registers whose target bit is 1
+            } else {
+                op.rmask = 0;
+                op.fl = 0;
             }
             if (op_is_ip(h.kind) && h.slot + 1 > plan->n_slots)
                 plan->n_slots = h.slot + 1;
             plan->ops.push_back(op);
             plan->op_pass.push_back(pass_index);
         }
+        (void)all_regs;
         D.op_end = (uint32_t)plan->ops.size();
         plan->phases.push_back(D);
-        prev = &plan->phases.back().lay;
+        prev = D.lay;
     }
     P.phase_end = (uint32_t)plan->phases.size();
-    const DevLayout &last = plan->phases.back().lay;
-    if (last_coalesced && layouts_equal(last, make_layout(phs.back().rs, pp, true), pp)) {
-        P.lay_st = last;
-        P.transpose_st = 0;
-    } else {
-        P.lay_st = make_layout(default_rs, pp, true);
-        P.transpose_st = layouts_equal(P.lay_st, last, pp) ? 0u : 1u;
+    const DevLayout last = prev;
+    const bool store_direct = last_coalesced && layouts_equal(last, make_layout(phs.back().rs, pp, true), pp);
+    P.lay_st = store_direct ? last : make_layout(default_rs, pp, true);
+    P.transpose_st = layouts_equal(P.lay_st, last, pp) ? 0u : 1u;
+    if (P.transpose_st) {
+        for (int q = 0; q < NR; q++)
+            P.st_src_lswz[q] = last.lswz[tab[q]];
+        reset_tab();
     }
-    // global offsets of the load/store layouts
-    for (int r = 0; r < (1 << pp.rlog); r++) {
+    // global offsets of the load/store layouts (the store through the final map)
+    for (int r = 0; r < NR; r++) {
         P.greg_ld[r] = deposit(P.lay_ld.lreg[r], P.qpos, pp.k);
-        P.greg_st[r] = deposit(P.lay_st.lreg[r], P.qpos, pp.k);
+        P.greg_st[r] = deposit(P.lay_st.lreg[tab[r]], P.qpos, pp.k);
     }
     for (int j = 0; j < pp.tlog; j++) {
         P.gthr_ld[j] = P.qpos[P.lay_ld.thr[j]];
@@ -547,6 +605,20 @@ static int check_layout(const DevLayout &L, const PlanParams &pp, const char *wh
     return DMV_OK;
 }
 
+// A transpose's write offsets must be a permutation of its source layout's.
+static int check_offsets(const uint32_t *src, const DevLayout &L, const PlanParams &pp, const char *what)
+{
+    const int NR = 1 << pp.rlog;
+    std::vector<uint32_t> a(src, src + NR), b(L.lswz, L.lswz + NR);
+    std::sort(a.begin(), a.end());
+    std::sort(b.begin(), b.end());
+    if (a != b) {
+        set_error("internal: %s offsets are not a permutation of the layout", what);
+        return DMV_ERR_STATE;
+    }
+    return DMV_OK;
+}
+
 int validate_plan(const Plan &plan, const PlanParams &pp)
 {
     const uint64_t n_amp = uint64_t(1) << pp.n;
@@ -588,72 +660,58 @@ int validate_plan(const Plan &plan, const PlanParams &pp)
                 set_error("internal: op range invalid");
                 return DMV_ERR_STATE;
             }
+            if (D.transpose_in)
+                DMV_TRY(check_offsets(D.src_lswz, p == P.phase_begin ? P.lay_ld : plan.phases[p - 1].lay, pp,
+                                      "transpose source"));
             for (uint32_t o = D.op_begin; o < D.op_end; o++) {
                 const DevOp &op = plan.ops[o];
+                const int NR = 1 << pp.rlog;
+                const uint32_t regs = NR >= 32 ? 0xFFFFFFFFu : ((1u << NR) - 1);
                 if (op.target >= (uint32_t)pp.k || (op.ctrl >> pp.k) != 0 || ((op.ctrl >> op.target) & 1)) {
                     set_error("internal: op bits out of range");
-                    return DMV_ERR_STATE;
-                }
-                if (op_needs_register(op.kind) &&
-                    (op.rbit >= (uint32_t)pp.rlog || D.lay.lreg[1u << op.rbit] != (1u << op.target))) {
-                    set_error("internal: op register bit inconsistent");
-                    return DMV_ERR_STATE;
-                }
-                const bool reg_ok = op.rbit < (uint32_t)pp.rlog && D.lay.lreg[1u << op.rbit] == (1u << op.target);
-                const bool thr_ok = op.rbit == kNoReg && op.tbit < (uint32_t)pp.tlog && D.lay.thr[op.tbit] == op.target;
-                if (!reg_ok && !thr_ok) {
-                    set_error("internal: op target bit not in the phase layout");
-                    return DMV_ERR_STATE;
-                }
-                uint64_t cm = 0;
-                for (int j = 0; j < 32; j++) {
-                    if ((op.ctrl_reg >> j) & 1) {
-                        if (j >= pp.rlog)
-                            return set_error("internal: register control out of range"), DMV_ERR_STATE;
-                        cm |= D.lay.lreg[1u << j];
-                    }
-                    if ((op.ctrl_thr >> j) & 1) {
-                        if (j >= pp.tlog)
-                            return set_error("internal: thread control out of range"), DMV_ERR_STATE;
-                        cm |= uint64_t(1) << D.lay.thr[j];
-                    }
-                }
-                if (cm != op.ctrl) {
-                    set_error("internal: split controls do not match the control mask");
                     return DMV_ERR_STATE;
                 }
                 if (op.kind != OP_GEN && op.kind != OP_DIAG && op.kind != OP_SWAP && !op_is_ip(op.kind)) {
                     set_error("internal: op kind invalid");
                     return DMV_ERR_STATE;
                 }
+                if ((op.en & ~regs) || (op.fl & ~regs) || (op.ctrl_thr >> pp.tlog) != 0) {
+                    set_error("internal: op masks out of range");
+                    return DMV_ERR_STATE;
+                }
+                const bool target_reg = register_bit(D.lay, pp, op.target) != kNoReg;
+                if (op_needs_register(op.kind)) {
+                    if (!target_reg || op.rmask == 0 || (op.rmask & ~regs) || op.tbit != kNoReg) {
+                        set_error("internal: pair op without a register partner");
+                        return DMV_ERR_STATE;
+                    }
+                } else if (target_reg) {
+                    if ((op.rmask & ~regs) || op.tbit != kNoReg) {
+                        set_error("internal: register diagonal masks out of range");
+                        return DMV_ERR_STATE;
+                    }
+                } else if (op.tbit >= (uint32_t)pp.tlog || D.lay.thr[op.tbit] != op.target) {
+                    set_error("internal: thread-bit target inconsistent");
+                    return DMV_ERR_STATE;
+                }
+                for (int j = 0; j < pp.tlog; j++)
+                    if (((op.ctrl_thr >> j) & 1) && !((op.ctrl >> D.lay.thr[j]) & 1)) {
+                        set_error("internal: thread control not in the control mask");
+                        return DMV_ERR_STATE;
+                    }
             }
         }
+        if (P.transpose_st)
+            DMV_TRY(check_offsets(P.st_src_lswz, plan.phases[P.phase_end - 1].lay, pp, "store transpose source"));
     }
     return DMV_OK;
 }
 
 std::vector<HostOp> plan_global_ops(const Plan &plan, std::vector<int> *pass_of_op)
 {
-    std::vector<HostOp> out;
-    for (size_t pi = 0; pi < plan.passes.size(); pi++) {
-        const DevPass &P = plan.passes[pi];
-        for (uint32_t p = P.phase_begin; p < P.phase_end; p++) {
-            const DevPhase &D = plan.phases[p];
-            for (uint32_t o = D.op_begin; o < D.op_end; o++) {
-                const DevOp &d = plan.ops[o];
-                HostOp h;
-                h.kind = d.kind;
-                h.target = (int)P.qpos[d.target];
-                h.ctrl = deposit(d.ctrl, P.qpos, (int)P.k);
-                std::memcpy(h.m, d.m, sizeof(h.m));
-                h.slot = d.slot;
-                out.push_back(h);
-                if (pass_of_op)
-                    pass_of_op->push_back((int)pi);
-            }
-        }
-    }
-    return out;
+    if (pass_of_op)
+        *pass_of_op = plan.logical_pass;
+    return plan.logical;
 }
 
 std::vector<DevOp> global_dev_ops(const std::vector<HostOp> &ops)

@@ -26,6 +26,9 @@ struct HostOp {
     int slot;  // parameter index of an IP op, else -1
 };
 
+// Process-wide tunables (defined by the runtime; environment DMV_<NAME> or dmv_tune_set).
+int64_t tune_value(const char *name, int64_t dflt);
+
 bool gate_is_parametric(int kind);
 int gate_num_params(int kind);
 // Row-major 2x2 of a gate, as m00r m00i m01r m01i m10r m10i m11r m11i.
@@ -59,6 +62,10 @@ struct Plan {
     std::vector<DevOp> ops;      // tile-local ops, grouped by phase
     std::vector<int> op_pass;    // pass of each op
     int n_slots = 0;             // 1 + largest IP slot
+    // Every op in execution order with global qubits, including the X/CNOT
+    // ops that only relabel registers and emit no device op.
+    std::vector<HostOp> logical;
+    std::vector<int> logical_pass;
 };
 
 // Greedy scheduler: packs ops into passes of <= k qubits (always including the

@@ -49,6 +49,7 @@ struct Context {
     double ms[DMV_KC_COUNT] = {};
     double bytes[DMV_KC_COUNT] = {};
     std::map<std::string, int64_t> tune;
+    uint64_t tune_epoch = 0;  // bumped by every tune_set: cached plans rebuild
 };
 
 static Context &ctx()
@@ -327,7 +328,7 @@ int prepare_plan(DevicePlan &dp, uint64_t version, int n, int dtype, bool adjoin
     PlanParams pp = small ? PlanParams{} : tile_params(n, dtype, adjoint);
     const bool same_shape = dp.valid && dp.dtype == dtype && dp.small == small &&
                             (small || (dp.pp.k == pp.k && dp.pp.tlog == pp.tlog && dp.pp.max_gen_per_pass == pp.max_gen_per_pass));
-    if (same_shape && dp.version == version)
+    if (same_shape && dp.version == version && dp.epoch == tune_epoch())
         return DMV_OK;
     std::vector<HostOp> merged = merge_ops(ops);
     dp.small = small;
@@ -349,6 +350,7 @@ int prepare_plan(DevicePlan &dp, uint64_t version, int n, int dtype, bool adjoin
         DMV_TRY(upload(dp.d_ops, dp.cap_ops, dp.plan.ops));
     }
     dp.version = version;
+    dp.epoch = tune_epoch();
     dp.build_id++;
     dp.valid = true;
     return DMV_OK;
@@ -605,6 +607,13 @@ int profile_get(int kc, int64_t *launches, double *ms, double *bytes)
     return DMV_OK;
 }
 
-void tune_set(const char *name, int64_t v) { ctx().tune[name] = v; }
+void tune_set(const char *name, int64_t v)
+{
+    Context &C = ctx();
+    C.tune[name] = v;
+    C.tune_epoch++;
+}
+
+uint64_t tune_epoch() { return ctx().tune_epoch; }
 
 }  // namespace dmv

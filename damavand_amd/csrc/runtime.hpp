@@ -26,6 +26,7 @@ struct DevicePlan {
     bool valid = false;
     uint64_t version = ~uint64_t(0);
     uint64_t build_id = 0;        // bumped on every rebuild
+    uint64_t epoch = 0;           // tuning epoch the plan was built under
     int dtype = -1;
     bool small = false;           // whole-state kernel instead of tile passes
     PlanParams pp{};
@@ -65,6 +66,7 @@ void release(void *p);
 int copy_h2d(void *dst, const void *src, uint64_t bytes);
 int copy_d2h(void *dst, const void *src, uint64_t bytes);
 int64_t tune_value(const char *name, int64_t dflt);
+uint64_t tune_epoch();
 
 // Tile shape for a forward (adjoint = false) or reverse sweep on n qubits.
 PlanParams tile_params(int n, int dtype, bool adjoint);

@@ -1,10 +1,16 @@
 // Device building blocks of the fused tile executor, shared by the big-state
 // pass kernels, the whole-state kernels and the batched persistent kernel.
 //
-// Every per-op condition is resolved on the host into a register part (a
-// mask over the register index: uniform, so the per-pair checks compile to
-// scalar branches) and a thread part (a mask over the thread id: one
-// predicate per op), so the vector ALU only does the arithmetic.
+// The host resolves every op against the phase's register/thread layout and
+// against the register renaming in force (see make_pass in plan.cpp):
+//  * pair ops touch physical registers p and p ^ rmask; the pair loop is
+//    generated per mask value, so register indices stay compile-time;
+//  * which member of a pair is the target-bit-0 one (fl), which registers
+//    pass their register-bit controls (en) and which registers of a
+//    register-target diagonal have target bit 1 (rmask) are uniform masks,
+//    so choosing coefficients costs scalar selects only;
+//  * thread-bit controls are one predicate per op.
+// The vector ALU therefore does only the complex arithmetic.
 #pragma once
 
 #include "device.hpp"
@@ -29,17 +35,20 @@ template <int TLOG> __device__ __forceinline__ uint64_t thread_global_offset(con
     return o;
 }
 
-// Moves the tile from layout A to layout B through LDS.  The swizzle is
-// linear, so the address of (thread, register) is swz(thread part) ^ lswz[r].
+// Moves the tile through LDS: register p of thread t is written at
+// swz(thread part in A) ^ src[p] and register r is read back from
+// swz(thread part in B) ^ B->lswz[r].  The swizzle is linear, so these are
+// the swizzled local indices of (t, p) in A and (t, r) in B.
 template <class V, int NR, int TLOG>
-__device__ __forceinline__ void transpose(V (&v)[NR], V *lds, const DevLayout *A, const DevLayout *B, uint32_t tid)
+__device__ __forceinline__ void transpose(V (&v)[NR], V *lds, const DevLayout *A, const uint32_t *src,
+                                          const DevLayout *B, uint32_t tid)
 {
     constexpr int G = sizeof(V) == 16 ? 3 : 4;
     char *base = reinterpret_cast<char *>(lds);
     const uint32_t la = swz_index<G>(thread_local_offset<TLOG>(A, tid)) * (uint32_t)sizeof(V);
 #pragma unroll
     for (int r = 0; r < NR; r++)
-        *reinterpret_cast<V *>(base + (la ^ A->lswz[r])) = v[r];
+        *reinterpret_cast<V *>(base + (la ^ src[r])) = v[r];
     __syncthreads();
     const uint32_t lb = swz_index<G>(thread_local_offset<TLOG>(B, tid)) * (uint32_t)sizeof(V);
 #pragma unroll
@@ -48,114 +57,119 @@ __device__ __forceinline__ void transpose(V (&v)[NR], V *lds, const DevLayout *A
     __syncthreads();
 }
 
-// The pair loops are straight-line code (no branch per pair) so that the
-// independent FP64 chains of all pairs interleave; register-bit controls, when
-// present, select results instead of branching.
-template <int J, class V, class R, int NR>
-__device__ __forceinline__ void op_gen(V (&v)[NR], const Mat<R> &M, uint32_t ctrl_reg)
+constexpr int highest_bit(int m) { return m >= 16 ? 16 : m >= 8 ? 8 : m >= 4 ? 4 : m >= 2 ? 2 : 1; }
+
+// Pair op on registers (p, p ^ M).  f: p holds the target-bit-1 member, so
+// the matrix is applied with rows and columns exchanged.
+template <int M, class V, class R, int NR>
+__device__ __forceinline__ void op_gen_m(V (&v)[NR], const Mat<R> &A, uint32_t en, uint32_t fl)
 {
-    constexpr int bit = 1 << J;
-    if (ctrl_reg == 0) {
+    constexpr int HB = highest_bit(M);
+    const uint32_t all = NR >= 32 ? 0xFFFFFFFFu : ((1u << NR) - 1);
+    if (en == all) {
 #pragma unroll
-        for (int a = 0; a < NR; a++)
-            if (!(a & bit))
-                apply2(v[a], v[a | bit], M);
+        for (int p = 0; p < NR; p++) {
+            if (p & HB)
+                continue;
+            const bool f = (fl >> p) & 1u;
+            Mat<R> B;
+            B.m00r = f ? A.m11r : A.m00r; B.m00i = f ? A.m11i : A.m00i;
+            B.m01r = f ? A.m10r : A.m01r; B.m01i = f ? A.m10i : A.m01i;
+            B.m10r = f ? A.m01r : A.m10r; B.m10i = f ? A.m01i : A.m10i;
+            B.m11r = f ? A.m00r : A.m11r; B.m11i = f ? A.m00i : A.m11i;
+            apply2(v[p], v[p ^ M], B);
+        }
     } else {
 #pragma unroll
-        for (int a = 0; a < NR; a++) {
-            if (a & bit)
+        for (int p = 0; p < NR; p++) {
+            if (p & HB)
                 continue;
-            V x = v[a], y = v[a | bit];
-            apply2(x, y, M);
-            const bool ok = ((uint32_t)a & ctrl_reg) == ctrl_reg;
-            v[a] = ok ? x : v[a];
-            v[a | bit] = ok ? y : v[a | bit];
+            const bool f = (fl >> p) & 1u;
+            Mat<R> B;
+            B.m00r = f ? A.m11r : A.m00r; B.m00i = f ? A.m11i : A.m00i;
+            B.m01r = f ? A.m10r : A.m01r; B.m01i = f ? A.m10i : A.m01i;
+            B.m10r = f ? A.m01r : A.m10r; B.m10i = f ? A.m01i : A.m10i;
+            B.m11r = f ? A.m00r : A.m11r; B.m11i = f ? A.m00i : A.m11i;
+            V x = v[p], y = v[p ^ M];
+            apply2(x, y, B);
+            const bool ok = (en >> p) & 1u;
+            v[p] = ok ? x : v[p];
+            v[p ^ M] = ok ? y : v[p ^ M];
         }
     }
 }
 
-template <int J, class V, int NR> __device__ __forceinline__ void op_swap(V (&v)[NR], uint32_t ctrl_reg)
+template <int M, class V, int NR> __device__ __forceinline__ void op_swap_m(V (&v)[NR], uint32_t en)
 {
-    constexpr int bit = 1 << J;
+    constexpr int HB = highest_bit(M);
 #pragma unroll
-    for (int a = 0; a < NR; a++) {
-        if (a & bit)
+    for (int p = 0; p < NR; p++) {
+        if (p & HB)
             continue;
-        const bool ok = ((uint32_t)a & ctrl_reg) == ctrl_reg;
-        const V x = v[a], y = v[a | bit];
-        v[a] = ok ? y : x;
-        v[a | bit] = ok ? x : y;
+        const bool ok = (en >> p) & 1u;
+        const V x = v[p], y = v[p ^ M];
+        v[p] = ok ? y : x;
+        v[p ^ M] = ok ? x : y;
     }
 }
 
-template <int J, class V, class R, int NR>
-__device__ __forceinline__ void op_diag_reg(V (&v)[NR], R d0r, R d0i, R d1r, R d1i, uint32_t ctrl_reg)
+#define DMV_MASK_CASES(X)                                                                          \
+    X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17)   \
+    X(18) X(19) X(20) X(21) X(22) X(23) X(24) X(25) X(26) X(27) X(28) X(29) X(30) X(31)
+
+template <class V, class R, int NR>
+__device__ __forceinline__ void op_pair(V (&v)[NR], const DevOp *op)
 {
-    constexpr int bit = 1 << J;
-#pragma unroll
-    for (int a = 0; a < NR; a++) {
-        if (a & bit)
-            continue;
-        const bool ok = ((uint32_t)a & ctrl_reg) == ctrl_reg;
-        const V x = cmul(v[a], d0r, d0i), y = cmul(v[a | bit], d1r, d1i);
-        v[a] = ok ? x : v[a];
-        v[a | bit] = ok ? y : v[a | bit];
+    const uint32_t en = op->en, fl = op->fl;
+    if (op->kind == OP_GEN) {
+        const Mat<R> A = load_mat<R>(op->m);
+        switch (op->rmask) {
+#define DMV_GEN_CASE(M)                                                                            \
+    case M:                                                                                        \
+        if constexpr (M < NR)                                                                      \
+            op_gen_m<M>(v, A, en, fl);                                                             \
+        break;
+            DMV_MASK_CASES(DMV_GEN_CASE)
+#undef DMV_GEN_CASE
+        default: break;
+        }
+    } else {
+        switch (op->rmask) {
+#define DMV_SWAP_CASE(M)                                                                           \
+    case M:                                                                                        \
+        if constexpr (M < NR)                                                                      \
+            op_swap_m<M>(v, en);                                                                   \
+        break;
+            DMV_MASK_CASES(DMV_SWAP_CASE)
+#undef DMV_SWAP_CASE
+        default: break;
+        }
     }
 }
 
-// diag(m00, m11) on the op's target, which is a register bit (rbit) or a
-// thread bit (tbit).
+// diag(m00, m11) on a register-bit target (rmask: registers with target bit
+// 1) or on a thread-bit target (tbit), on the registers in en.
 template <class V, class R, int NR>
 __device__ __forceinline__ void op_diag(V (&v)[NR], const DevOp *op, uint32_t tid)
 {
     const R d0r = (R)op->m[0], d0i = (R)op->m[1], d1r = (R)op->m[6], d1i = (R)op->m[7];
-    const uint32_t ctrl_reg = op->ctrl_reg;
-    const uint32_t rb = op->rbit;
-    if (rb != kNoReg) {
-        switch (rb) {
-        case 0: if constexpr (NR > 1) op_diag_reg<0>(v, d0r, d0i, d1r, d1i, ctrl_reg); break;
-        case 1: if constexpr (NR > 2) op_diag_reg<1>(v, d0r, d0i, d1r, d1i, ctrl_reg); break;
-        case 2: if constexpr (NR > 4) op_diag_reg<2>(v, d0r, d0i, d1r, d1i, ctrl_reg); break;
-        case 3: if constexpr (NR > 8) op_diag_reg<3>(v, d0r, d0i, d1r, d1i, ctrl_reg); break;
-        case 4: if constexpr (NR > 16) op_diag_reg<4>(v, d0r, d0i, d1r, d1i, ctrl_reg); break;
-        default: break;
+    const uint32_t en = op->en;
+    const uint32_t all = NR >= 32 ? 0xFFFFFFFFu : ((1u << NR) - 1);
+    if (op->tbit == kNoReg) {
+        const uint32_t ones = op->rmask;
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            const bool one = (ones >> r) & 1u;
+            const V x = cmul(v[r], one ? d1r : d0r, one ? d1i : d0i);
+            v[r] = (en == all || ((en >> r) & 1u)) ? x : v[r];
         }
     } else {
         const bool one = (tid >> op->tbit) & 1u;
         const R dr = one ? d1r : d0r, di = one ? d1i : d0i;
 #pragma unroll
         for (int r = 0; r < NR; r++) {
-            const bool ok = ((uint32_t)r & ctrl_reg) == ctrl_reg;
             const V x = cmul(v[r], dr, di);
-            v[r] = ok ? x : v[r];
-        }
-    }
-}
-
-// Applies one GEN or SWAP op whose target is register bit op->rbit.
-template <class V, class R, int RLOG>
-__device__ __forceinline__ void op_register(V (&v)[1 << RLOG], const DevOp *op)
-{
-    const uint32_t ctrl_reg = op->ctrl_reg;
-    const uint32_t j = op->rbit;
-    if (op->kind == OP_GEN) {
-        const Mat<R> M = load_mat<R>(op->m);
-        switch (j) {
-        case 0: if constexpr (RLOG > 0) op_gen<0>(v, M, ctrl_reg); break;
-        case 1: if constexpr (RLOG > 1) op_gen<1>(v, M, ctrl_reg); break;
-        case 2: if constexpr (RLOG > 2) op_gen<2>(v, M, ctrl_reg); break;
-        case 3: if constexpr (RLOG > 3) op_gen<3>(v, M, ctrl_reg); break;
-        case 4: if constexpr (RLOG > 4) op_gen<4>(v, M, ctrl_reg); break;
-        default: break;
-        }
-    } else if (op->kind == OP_SWAP) {
-        switch (j) {
-        case 0: if constexpr (RLOG > 0) op_swap<0>(v, ctrl_reg); break;
-        case 1: if constexpr (RLOG > 1) op_swap<1>(v, ctrl_reg); break;
-        case 2: if constexpr (RLOG > 2) op_swap<2>(v, ctrl_reg); break;
-        case 3: if constexpr (RLOG > 3) op_swap<3>(v, ctrl_reg); break;
-        case 4: if constexpr (RLOG > 4) op_swap<4>(v, ctrl_reg); break;
-        default: break;
+            v[r] = (en == all || ((en >> r) & 1u)) ? x : v[r];
         }
     }
 }
@@ -177,7 +191,7 @@ __device__ __forceinline__ void run_ops(V (&v)[1 << RLOG], const DevOp *__restri
         if (op->kind == OP_DIAG)
             op_diag<V, R>(v, op, tid);
         else
-            op_register<V, R, RLOG>(v, op);
+            op_pair<V, R>(v, op);
     }
 }
 
@@ -221,12 +235,12 @@ __device__ __forceinline__ void process_tile(typename Cplx<R>::T (&v)[1 << RLOG]
     for (uint32_t ph = P->phase_begin; ph < P->phase_end; ph++) {
         const DevPhase *D = phases + ph;
         if (D->transpose_in)
-            transpose<V, NR, TLOG>(v, lds, cur, &D->lay, tid);
+            transpose<V, NR, TLOG>(v, lds, cur, D->src_lswz, &D->lay, tid);
         cur = &D->lay;
         run_ops<V, R, RLOG>(v, ops, D->op_begin, D->op_end, tid);
     }
     if (P->transpose_st)
-        transpose<V, NR, TLOG>(v, lds, cur, &P->lay_st, tid);
+        transpose<V, NR, TLOG>(v, lds, cur, P->st_src_lswz, &P->lay_st, tid);
 }
 
 // One tile of a forward pass: load, run the phases, store.
@@ -284,21 +298,24 @@ __device__ __forceinline__ void tile_stream(typename Cplx<R>::T *__restrict__ st
 }
 
 // --------------------------------------------------------------- adjoint --
-// Im<lambda|P|psi> over the thread's register pairs for P = X or Y on bit J.
-template <int J, class V, int NR>
-__device__ __forceinline__ double ip_pair(const V (&p)[NR], const V (&l)[NR], uint32_t kind)
+// Im<lambda|P|psi> over register pairs (p, p ^ M) for P = X or Y; for Y the
+// sign follows which member holds target bit 0.
+template <int M, class V, int NR>
+__device__ __forceinline__ double ip_pair_m(const V (&p)[NR], const V (&l)[NR], uint32_t kind, uint32_t fl)
 {
+    constexpr int HB = highest_bit(M);
     double acc = 0.0;
 #pragma unroll
     for (int a = 0; a < NR; a++) {
-        if (a & (1 << J))
+        if (a & HB)
             continue;
-        constexpr int bit = 1 << J;
-        const V pa = p[a], pb = p[a | bit], la = l[a], lb = l[a | bit];
-        if (kind == OP_IPX)  // conj(la) pb + conj(lb) pa
+        const V pa = p[a], pb = p[a ^ M], la = l[a], lb = l[a ^ M];
+        if (kind == OP_IPX) {  // conj(la) pb + conj(lb) pa: symmetric in the pair
             acc += (double)la.x * pb.y - (double)la.y * pb.x + (double)lb.x * pa.y - (double)lb.y * pa.x;
-        else  // (Y psi)_a = -i psi_b, (Y psi)_b = i psi_a
-            acc += -((double)la.x * pb.x + (double)la.y * pb.y) + ((double)lb.x * pa.x + (double)lb.y * pa.y);
+        } else {  // (Y psi)_0 = -i psi_1, (Y psi)_1 = i psi_0
+            const double w = -((double)la.x * pb.x + (double)la.y * pb.y) + ((double)lb.x * pa.x + (double)lb.y * pa.y);
+            acc += ((fl >> a) & 1u) ? -w : w;
+        }
     }
     return acc;
 }
@@ -310,22 +327,25 @@ __device__ __forceinline__ double ip_registers(const V (&p)[1 << RLOG], const V 
     constexpr int NR = 1 << RLOG;
     double acc = 0.0;
     if (op->kind == OP_IPZ) {
-        const uint32_t rb = op->rbit;
-        const bool thread_one = rb == kNoReg && ((tid >> op->tbit) & 1u);
+        const bool reg = op->tbit == kNoReg;
+        const bool thread_one = !reg && ((tid >> op->tbit) & 1u);
+        const uint32_t ones = op->rmask;
 #pragma unroll
         for (int r = 0; r < NR; r++) {
-            const bool one = rb != kNoReg ? (((uint32_t)r >> rb) & 1u) : thread_one;
+            const bool one = reg ? ((ones >> r) & 1u) : thread_one;
             const double w = (double)l[r].x * p[r].y - (double)l[r].y * p[r].x;
             acc += one ? -w : w;
         }
         return acc;
     }
-    switch (op->rbit) {
-    case 0: if constexpr (RLOG > 0) acc = ip_pair<0>(p, l, op->kind); break;
-    case 1: if constexpr (RLOG > 1) acc = ip_pair<1>(p, l, op->kind); break;
-    case 2: if constexpr (RLOG > 2) acc = ip_pair<2>(p, l, op->kind); break;
-    case 3: if constexpr (RLOG > 3) acc = ip_pair<3>(p, l, op->kind); break;
-    case 4: if constexpr (RLOG > 4) acc = ip_pair<4>(p, l, op->kind); break;
+    switch (op->rmask) {
+#define DMV_IP_CASE(M)                                                                             \
+    case M:                                                                                        \
+        if constexpr (M < NR)                                                                      \
+            acc = ip_pair_m<M>(p, l, op->kind, op->fl);                                            \
+        break;
+        DMV_MASK_CASES(DMV_IP_CASE)
+#undef DMV_IP_CASE
     default: break;
     }
     return acc;
@@ -352,8 +372,8 @@ __device__ __forceinline__ void tile_body_adjoint(typename Cplx<R>::T *__restric
     for (uint32_t ph = P->phase_begin; ph < P->phase_end; ph++) {
         const DevPhase *D = phases + ph;
         if (D->transpose_in) {
-            transpose<V, NR, TLOG>(p, lds, cur, &D->lay, tid);
-            transpose<V, NR, TLOG>(l, lds, cur, &D->lay, tid);
+            transpose<V, NR, TLOG>(p, lds, cur, D->src_lswz, &D->lay, tid);
+            transpose<V, NR, TLOG>(l, lds, cur, D->src_lswz, &D->lay, tid);
         }
         cur = &D->lay;
         for (uint32_t o = D->op_begin; o < D->op_end; o++) {
@@ -370,14 +390,14 @@ __device__ __forceinline__ void tile_body_adjoint(typename Cplx<R>::T *__restric
                 op_diag<V, R>(p, op, tid);
                 op_diag<V, R>(l, op, tid);
             } else {
-                op_register<V, R, RLOG>(p, op);
-                op_register<V, R, RLOG>(l, op);
+                op_pair<V, R>(p, op);
+                op_pair<V, R>(l, op);
             }
         }
     }
     if (P->transpose_st) {
-        transpose<V, NR, TLOG>(p, lds, cur, &P->lay_st, tid);
-        transpose<V, NR, TLOG>(l, lds, cur, &P->lay_st, tid);
+        transpose<V, NR, TLOG>(p, lds, cur, P->st_src_lswz, &P->lay_st, tid);
+        transpose<V, NR, TLOG>(l, lds, cur, P->st_src_lswz, &P->lay_st, tid);
     }
     const uint64_t gs = g | thread_global_offset<TLOG>(P->gthr_st, tid);
     store_tile<false>(p, psi, gs, P->greg_st);

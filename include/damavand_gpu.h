@@ -234,6 +234,15 @@ int dmv_circuit_plan_stats(dmv_circuit *c, int dtype, int *passes, int *phases, 
 int dmv_circuit_plan_ops(dmv_circuit *c, int dtype, int tile_qubits, int *kind, int *target,
                          uint64_t *control_mask, double *m, int *pass, int capacity, int *count);
 
+/* The complete device plan (passes, phases with their register/thread
+ * layouts, ops with their masks) as JSON text, for checking the tile
+ * executor's index arithmetic on the CPU.  adjoint selects the reverse-sweep
+ * plan; tile_qubits / threads_log2 / max_gen_per_pass of 0 take the tuned
+ * defaults.  Writes at most `capacity` bytes (NUL-terminated) and the full
+ * length into *length.  No device needed. */
+int dmv_circuit_plan_json(dmv_circuit *c, int dtype, int adjoint, int tile_qubits, int threads_log2,
+                          int max_gen_per_pass, char *buffer, uint64_t capacity, uint64_t *length);
+
 /* ---- batched forward + adjoint: many small circuits in one persistent launch ---- */
 /* Circuit i has n[i] qubits and the gates gate_kind/control/target/param in
  * [gate_offset[i], gate_offset[i+1]); its observable is the Z/ZZ chain

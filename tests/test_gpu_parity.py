@@ -100,6 +100,26 @@ def test_pipelined_tile_passes(native, n, grid, nt):
     assert err <= TOL
 
 
+@pytest.mark.parametrize("n,k,tlog,gen", [(20, 13, 9, 8), (19, 12, 9, 4), (20, 11, 8, 40), (22, 12, 8, 2)])
+def test_tile_shapes(native, n, k, tlog, gen):
+    """Every tile shape the tuner may pick (512- and 256-thread workgroups,
+    11- to 13-qubit tiles, different compute budgets per pass)."""
+    say(f"case tile shape n={n} k={k} tlog={tlog} gen={gen}")
+    rng = np.random.default_rng(400 + n + k)
+    gates = random_gates(rng, n, 300)
+    c = build_circuit(n, gates)
+    for key, v in (("tile_qubits", k), ("tile_threads_log2", tlog), ("max_gen_per_pass", gen)):
+        N.tune_set(key, v)
+    try:
+        c.forward()
+    finally:
+        for key in ("tile_qubits", "tile_threads_log2", "max_gen_per_pass"):
+            N.tune_set(key, 0 if key != "max_gen_per_pass" else 8)
+    err = np.max(np.abs(c.retrieve_amplitudes_on_host() - oracle.run_gates(n, gates).amp))
+    say(f"  passes={c.plan_stats()[0]} max-abs error {err:.3e}")
+    assert err <= TOL
+
+
 @pytest.mark.parametrize("n", [3, 12, 16])
 def test_random_circuit_per_gate(native, n):
     say(f"case random per-gate n={n}")

@@ -1,0 +1,123 @@
+"""The tile executor's plans, emulated on the CPU exactly as the kernels
+index them (tests/tile_emulator.py), against the oracle."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from damavand_amd import _native as N
+from damavand_amd import workloads as wl
+from tests.libhelpers import NativeCircuit
+from tests.tile_emulator import TileEmulator, plan_json
+
+
+def random_gates(rng, n, count):
+    kinds = [wl.GATE_H, wl.GATE_X, wl.GATE_Y, wl.GATE_Z, wl.GATE_RX, wl.GATE_RY, wl.GATE_RZ, wl.GATE_S, wl.GATE_T,
+             wl.GATE_CNOT, wl.GATE_CNOT, wl.GATE_CNOT, wl.GATE_U3, wl.GATE_CZ]
+    out = []
+    for _ in range(count):
+        k = kinds[rng.integers(len(kinds))]
+        t = int(rng.integers(n))
+        c = -1
+        if k in (wl.GATE_CNOT, wl.GATE_CZ):
+            c = int(rng.integers(n - 1))
+            c += c >= t
+        out.append((k, c, t, tuple(rng.uniform(0, 2 * math.pi, 3))))
+    return out
+
+
+def zero_state(n):
+    s = np.zeros(1 << n, dtype=complex)
+    s[0] = 1
+    return s
+
+
+@pytest.mark.parametrize("n,k,tlog,gen,seed", [
+    (13, 12, 8, 8, 1), (14, 12, 8, 4, 2), (15, 11, 8, 8, 3), (14, 9, 6, 8, 4), (15, 13, 9, 8, 5),
+    (16, 12, 9, 40, 6), (14, 10, 8, 2, 7), (13, 12, 8, 100, 8)])
+def test_forward_plan_emulation(n, k, tlog, gen, seed):
+    rng = np.random.default_rng(seed)
+    gates = random_gates(rng, n, 250)
+    nc = NativeCircuit(n, gates)
+    plan = plan_json(nc.h, tile_qubits=k, tlog=tlog, max_gen=gen)
+    got = TileEmulator(plan).forward(zero_state(n))
+    np.testing.assert_allclose(got, oracle.run_gates(n, gates).amp, atol=1e-12)
+
+
+def test_layered_workload_plan_emulation():
+    n = 16
+    gates = wl.layered_circuit(n, 3, 21)
+    nc = NativeCircuit(n, gates)
+    plan = plan_json(nc.h, tile_qubits=12, tlog=8, max_gen=8)
+    got = TileEmulator(plan).forward(zero_state(n))
+    np.testing.assert_allclose(got, oracle.run_gates(n, gates).amp, atol=1e-12)
+
+
+def test_complex64_plan_emulation():
+    n = 15
+    rng = np.random.default_rng(9)
+    gates = random_gates(rng, n, 200)
+    nc = NativeCircuit(n, gates)
+    plan = plan_json(nc.h, dtype=N.C64, tile_qubits=12, tlog=8, max_gen=8)
+    got = TileEmulator(plan, elem=8).forward(zero_state(n))
+    np.testing.assert_allclose(got, oracle.run_gates(n, gates).amp, atol=1e-12)
+
+
+def test_renaming_removes_register_cnots():
+    """CNOT ladders among register qubits emit no device op (only the map changes)."""
+    n = 14
+    gates = [(wl.GATE_H, -1, q, ()) for q in range(n)] + [(wl.GATE_CNOT, q, q + 1, ()) for q in range(n - 1)]
+    nc = NativeCircuit(n, gates)
+    plan = plan_json(nc.h, tile_qubits=12, tlog=8, max_gen=100)
+    swaps = sum(1 for P in plan["passes"] for ph in P["phases"] for op in ph["ops"] if op["kind"] == 2)
+    assert swaps < n - 1
+    got = TileEmulator(plan).forward(zero_state(n))
+    np.testing.assert_allclose(got, oracle.run_gates(n, gates).amp, atol=1e-12)
+
+
+def test_planted_mask_mistake_is_caught():
+    n = 13
+    rng = np.random.default_rng(11)
+    gates = random_gates(rng, n, 120)
+    nc = NativeCircuit(n, gates)
+    plan = plan_json(nc.h, tile_qubits=12, tlog=8, max_gen=100)
+    for P in plan["passes"]:
+        for ph in P["phases"]:
+            for op in ph["ops"]:
+                if op["kind"] == 0 and op["fl"]:
+                    op["fl"] ^= 1  # flip the member assignment of one pair
+                    break
+    got = TileEmulator(plan).forward(zero_state(n))
+    assert np.max(np.abs(got - oracle.run_gates(n, gates).amp)) > 1e-6
+
+
+@pytest.mark.parametrize("n,layers,k,tlog", [(13, 2, 11, 8), (14, 2, 12, 8), (13, 3, 9, 6)])
+def test_adjoint_plan_emulation(n, layers, k, tlog):
+    """The reverse sweep (G^dagger on psi and lambda, IP partials) emulated,
+    against parameter shift on the oracle."""
+    gates, a, b = wl.ansatz_circuit(n, layers, 500 + n)
+    pairs = wl.chain_pairs(n)
+    nc = NativeCircuit(n, gates)
+    plan = plan_json(nc.h, adjoint=True, tile_qubits=k, tlog=tlog, max_gen=8)
+    st = oracle.run_gates(n, gates)
+    idx = np.arange(1 << n)
+    h = np.zeros(1 << n)
+    for q in range(n):
+        h += np.where((idx >> q) & 1, -a[q], a[q])
+    for (i, j), bb in zip(pairs, b):
+        h += np.where(((idx >> i) ^ (idx >> j)) & 1, -bb, bb)
+    psi0, lam, grads = TileEmulator(plan).reverse(st.amp, h * st.amp)
+    # psi is uncomputed back to |0>
+    np.testing.assert_allclose(psi0, zero_state(n), atol=1e-12)
+    pidx = [i for i, g in enumerate(gates) if g[0] in wl.PARAMETRIC]
+    ref = []
+    for gi in pidx:
+        vals = []
+        for s in (0.5 * math.pi, -0.5 * math.pi):
+            g2 = list(gates)
+            kk, c, t, p = g2[gi]
+            g2[gi] = (kk, c, t, (p[0] + s,))
+            vals.append(oracle.run_gates(n, g2).expect_zz(a, pairs, b))
+        ref.append(0.5 * (vals[0] - vals[1]))
+    np.testing.assert_allclose(grads, ref, atol=1e-10)

@@ -1,0 +1,206 @@
+"""CPU emulation of the fused tile executor (damavand_amd/csrc/tile_ops.hpp),
+driven by the exact plan the library would upload (dmv_circuit_plan_json).
+
+Test infrastructure: it checks the scheduler's layouts, LDS swizzle offsets,
+register renaming masks and store offsets on the CPU, index for index as the
+kernels use them, so a wrong offset or mask is caught before any GPU run.
+"""
+import ctypes
+import json
+
+import numpy as np
+
+from damavand_amd import _native as N
+
+NO_REG = 0xFFFFFFFF
+OP_GEN, OP_DIAG, OP_SWAP, OP_IPX, OP_IPY, OP_IPZ = 0, 1, 2, 4, 5, 6
+
+
+def plan_json(handle, dtype=N.C128, adjoint=False, tile_qubits=0, tlog=0, max_gen=0):
+    length = ctypes.c_uint64(0)
+    N.call("dmv_circuit_plan_json", handle, dtype, int(adjoint), tile_qubits, tlog, max_gen, None, 0,
+           ctypes.byref(length))
+    buf = ctypes.create_string_buffer(length.value)
+    N.call("dmv_circuit_plan_json", handle, dtype, int(adjoint), tile_qubits, tlog, max_gen, buf, length.value,
+           ctypes.byref(length))
+    return json.loads(buf.value.decode())
+
+
+def swz(li, G):
+    li = np.asarray(li, dtype=np.int64)
+    f = np.zeros_like(li)
+    s = G
+    while s < 16:
+        f ^= li >> s
+        s += G
+    return li ^ (f & ((1 << G) - 1))
+
+
+def insert_zero(x, pos):
+    low = x & ((1 << pos) - 1)
+    return ((x >> pos) << (pos + 1)) | low
+
+
+def highest_bit(m):
+    for b in (16, 8, 4, 2, 1):
+        if m >= b:
+            return b
+    return 1
+
+
+class TileEmulator:
+    def __init__(self, plan, elem=16):
+        self.plan = plan
+        self.n, self.k, self.rlog, self.tlog = plan["n"], plan["k"], plan["rlog"], plan["tlog"]
+        self.NR, self.NT = 1 << self.rlog, 1 << self.tlog
+        self.elem = elem
+        self.G = 3 if elem == 16 else 4
+        self.tids = np.arange(self.NT)
+
+    def thread_local(self, lay):
+        o = np.zeros(self.NT, dtype=np.int64)
+        for j, b in enumerate(lay["thr"]):
+            o |= ((self.tids >> j) & 1) << b
+        return o
+
+    def thread_global(self, gthr):
+        o = np.zeros(self.NT, dtype=np.int64)
+        for j, b in enumerate(gthr):
+            o |= ((self.tids >> j) & 1) << b
+        return o
+
+    def tile_bases(self, P):
+        t = np.arange(1 << (self.n - self.k), dtype=np.int64)
+        for q in P["qpos"]:
+            t = insert_zero(t, q)
+        return t
+
+    def transpose(self, v, A, src, B):
+        """v: [tiles, NT, NR]; writes register p at swz(lt_A)^src[p], reads r at swz(lt_B)^lswz_B[r]."""
+        tiles = v.shape[0]
+        lds = np.full((tiles, 1 << self.k), np.nan + 0j)
+        hits = np.zeros(1 << self.k, dtype=np.int64)
+        la = swz(self.thread_local(A), self.G)
+        for p in range(self.NR):
+            assert src[p] % self.elem == 0
+            addr = la ^ (src[p] // self.elem)
+            assert addr.max() < (1 << self.k)
+            lds[:, addr] = v[:, :, p]
+            np.add.at(hits, addr, 1)
+        assert np.all(hits == 1), "transpose writes are not a bijection"
+        lb = swz(self.thread_local(B), self.G)
+        out = np.empty_like(v)
+        for r in range(self.NR):
+            out[:, :, r] = lds[:, lb ^ (B["lswz"][r] // self.elem)]
+        assert not np.isnan(out.real).any()
+        return out
+
+    def apply(self, v, op, lam=None, sink=None):
+        tm = (self.tids & op["ctrl_thr"]) == op["ctrl_thr"]
+        kind = op["kind"]
+        m = op["m"]
+        m00, m01, m10, m11 = complex(m[0], m[1]), complex(m[2], m[3]), complex(m[4], m[5]), complex(m[6], m[7])
+        en, fl, rm = op["en"], op["fl"], op["rmask"]
+        if kind in (OP_IPX, OP_IPY, OP_IPZ):
+            p_, l_ = v, lam
+            acc = np.zeros(v.shape[0])
+            if kind == OP_IPZ:
+                for r in range(self.NR):
+                    if op["tbit"] == NO_REG:
+                        sign = -1.0 if (rm >> r) & 1 else 1.0
+                    else:
+                        sign = np.where((self.tids >> op["tbit"]) & 1, -1.0, 1.0)
+                    w = (l_[:, :, r].real * p_[:, :, r].imag - l_[:, :, r].imag * p_[:, :, r].real)
+                    acc += (sign * w).sum(axis=1)
+            else:
+                hb = highest_bit(rm)
+                for a in range(self.NR):
+                    if a & hb:
+                        continue
+                    b = a ^ rm
+                    pa, pb, la, lb = p_[:, :, a], p_[:, :, b], l_[:, :, a], l_[:, :, b]
+                    if kind == OP_IPX:
+                        w = la.real * pb.imag - la.imag * pb.real + lb.real * pa.imag - lb.imag * pa.real
+                    else:
+                        w = -(la.real * pb.real + la.imag * pb.imag) + (lb.real * pa.real + lb.imag * pa.imag)
+                        if (fl >> a) & 1:
+                            w = -w
+                    acc += w.sum(axis=1)
+            sink(op["slot"], acc)
+            return
+        if kind == OP_DIAG:
+            for r in range(self.NR):
+                if not (en >> r) & 1:
+                    continue
+                if op["tbit"] == NO_REG:
+                    d = m11 if (rm >> r) & 1 else m00
+                else:
+                    d = np.where((self.tids >> op["tbit"]) & 1, m11, m00)[None, :]
+                    d = np.broadcast_to(d, v.shape[:2])[:, tm]
+                v[:, tm, r] = v[:, tm, r] * d
+            return
+        hb = highest_bit(rm)
+        assert 0 < rm < self.NR
+        for p in range(self.NR):
+            if p & hb:
+                continue
+            q = p ^ rm
+            if not (en >> p) & 1:
+                continue
+            x, y = v[:, tm, p].copy(), v[:, tm, q].copy()
+            if kind == OP_SWAP:
+                v[:, tm, p], v[:, tm, q] = y, x
+                continue
+            a00, a01, a10, a11 = (m11, m10, m01, m00) if (fl >> p) & 1 else (m00, m01, m10, m11)
+            v[:, tm, p] = a00 * x + a01 * y
+            v[:, tm, q] = a10 * x + a11 * y
+
+    def run_pass(self, P, psi, lam=None, sink=None):
+        g = self.tile_bases(P)
+        gl = self.thread_global(P["gthr_ld"])
+        idx_ld = g[:, None, None] | gl[None, :, None] | np.array(P["greg_ld"], dtype=np.int64)[None, None, :]
+        assert idx_ld.max() < psi.size
+        v = psi[idx_ld].copy()
+        lv = lam[idx_ld].copy() if lam is not None else None
+        cur = P["lay_ld"]
+        for ph in P["phases"]:
+            if ph["transpose_in"]:
+                v = self.transpose(v, cur, ph["src_lswz"], ph["lay"])
+                if lv is not None:
+                    lv = self.transpose(lv, cur, ph["src_lswz"], ph["lay"])
+            cur = ph["lay"]
+            for op in ph["ops"]:
+                if op["kind"] in (OP_IPX, OP_IPY, OP_IPZ):
+                    self.apply(v, op, lam=lv, sink=sink)
+                else:
+                    self.apply(v, op)
+                    if lv is not None:
+                        self.apply(lv, op)
+        if P["transpose_st"]:
+            v = self.transpose(v, cur, P["st_src_lswz"], P["lay_st"])
+            if lv is not None:
+                lv = self.transpose(lv, cur, P["st_src_lswz"], P["lay_st"])
+        gs = self.thread_global(P["gthr_st"])
+        idx_st = g[:, None, None] | gs[None, :, None] | np.array(P["greg_st"], dtype=np.int64)[None, None, :]
+        flat = np.sort(idx_st.reshape(-1))
+        assert np.array_equal(flat, np.arange(psi.size)), "stores do not cover the state exactly once"
+        psi[idx_st] = v
+        if lam is not None:
+            lam[idx_st] = lv
+
+    def forward(self, psi):
+        psi = psi.copy()
+        for P in self.plan["passes"]:
+            self.run_pass(P, psi)
+        return psi
+
+    def reverse(self, psi, lam):
+        psi, lam = psi.copy(), lam.copy()
+        grads = np.zeros(self.plan["n_slots"])
+
+        def sink(slot, acc):
+            grads[slot] += acc.sum()
+
+        for P in self.plan["passes"]:
+            self.run_pass(P, psi, lam, sink)
+        return psi, lam, grads
