@@ -268,6 +268,10 @@ PlanParams tile_params(int n, int dtype, bool adjoint)
     PlanParams pp;
     pp.n = n;
     pp.low_bits = coalesce_bits(dtype);
+    // wider contiguous runs per lane group (tunable; fewer free tile qubits)
+    const int64_t runs = tune_value("coalesce_qubits", 0);
+    if (runs > pp.low_bits && runs <= 6)
+        pp.low_bits = (int)runs;
     pp.swz_bits = dtype == DMV_C64 ? 4 : 3;
     pp.max_gen_per_pass = (int)tune_value("max_gen_per_pass", 8);
     int k = (int)tune_value(adjoint ? "adjoint_tile_qubits" : "tile_qubits", 0);

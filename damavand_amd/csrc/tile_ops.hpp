@@ -260,8 +260,12 @@ __device__ __forceinline__ void tile_body(typename Cplx<R>::T *__restrict__ stat
 }
 
 // A workgroup's share of a pass, software pipelined: while tile t is
-// processed and stored, the loads of the workgroup's next tile are in flight
-// (two register buffers, alternating).
+// processed and stored, the loads of the workgroup's next tile are in flight.
+// The prefetch is unconditional (the last iteration re-reads its own tile into
+// the unused buffer), so the compiler's wait counts see the prefetch as the
+// newest loads on every path and the first transpose waits only for tile t.
+// One copy of the phase interpreter keeps the instruction footprint of the
+// non-pipelined kernel.
 template <class R, int RLOG, int TLOG, bool NT>
 __device__ __forceinline__ void tile_stream(typename Cplx<R>::T *__restrict__ state, const DevPass *P,
                                             const DevPhase *__restrict__ phases, const DevOp *__restrict__ ops,
@@ -280,20 +284,16 @@ __device__ __forceinline__ void tile_stream(typename Cplx<R>::T *__restrict__ st
     load_tile<NT>(a, state, tile_base<K>(P, t) | tl, P->greg_ld);
     for (;;) {
         const uint64_t t2 = t + stride;
-        if (t2 < n_tiles)
-            load_tile<NT>(b, state, tile_base<K>(P, t2) | tl, P->greg_ld);
+        const bool more = t2 < n_tiles;
+        load_tile<NT>(b, state, tile_base<K>(P, more ? t2 : t) | tl, P->greg_ld);
         process_tile<R, RLOG, TLOG>(a, P, phases, ops, lds);
         store_tile<NT>(a, state, tile_base<K>(P, t) | ts, P->greg_st);
-        if (t2 >= n_tiles)
+        if (!more)
             break;
-        const uint64_t t3 = t2 + stride;
-        if (t3 < n_tiles)
-            load_tile<NT>(a, state, tile_base<K>(P, t3) | tl, P->greg_ld);
-        process_tile<R, RLOG, TLOG>(b, P, phases, ops, lds);
-        store_tile<NT>(b, state, tile_base<K>(P, t2) | ts, P->greg_st);
-        if (t3 >= n_tiles)
-            break;
-        t = t3;
+#pragma unroll
+        for (int r = 0; r < NR; r++)
+            a[r] = b[r];
+        t = t2;
     }
 }
 

@@ -121,3 +121,21 @@ def test_adjoint_plan_emulation(n, layers, k, tlog):
             vals.append(oracle.run_gates(n, g2).expect_zz(a, pairs, b))
         ref.append(0.5 * (vals[0] - vals[1]))
     np.testing.assert_allclose(grads, ref, atol=1e-10)
+
+
+@pytest.mark.parametrize("runs", [4, 5, 6])
+def test_wider_coalescing_runs_emulation(runs):
+    """Plans that keep more low qubits in lanes at load/store time."""
+    n = 15
+    rng = np.random.default_rng(20 + runs)
+    gates = random_gates(rng, n, 200)
+    nc = NativeCircuit(n, gates)
+    N.tune_set("coalesce_qubits", runs)
+    try:
+        plan = plan_json(nc.h, tile_qubits=12, tlog=8, max_gen=8)
+    finally:
+        N.tune_set("coalesce_qubits", 0)
+    for P in plan["passes"]:
+        assert P["gthr_ld"][:runs] == list(range(runs)) and P["gthr_st"][:runs] == list(range(runs))
+    got = TileEmulator(plan).forward(zero_state(n))
+    np.testing.assert_allclose(got, oracle.run_gates(n, gates).amp, atol=1e-12)
