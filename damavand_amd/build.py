@@ -27,6 +27,10 @@ COMMON = [
     f"--offload-arch={ARCH}",
     # no packed FP32 instructions in device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # keep the per-mask switch cases of the tile interpreter apart: sinking
+    # their common code merges register-array accesses into pointer phis,
+    # which sends the tile's registers to scratch memory
+    "-mllvm", "-simplifycfg-sink-common=false",
     "-I", os.path.join(ROOT, "include"),
 ]
 

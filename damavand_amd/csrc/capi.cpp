@@ -707,8 +707,8 @@ int dmv_circuit_plan_json(dmv_circuit *c, int dtype, int adjoint, int tile_qubit
                 char buf[512];
                 snprintf(buf, sizeof(buf),
                          "{\"kind\":%u,\"target\":%u,\"rmask\":%u,\"tbit\":%u,\"slot\":%d,\"en\":%u,\"fl\":%u,"
-                         "\"ctrl_thr\":%u,\"ctrl\":%llu,\"m\":[%.17g,%.17g,%.17g,%.17g,%.17g,%.17g,%.17g,%.17g]}",
-                         op.kind, op.target, op.rmask, op.tbit, op.slot, op.en, op.fl, op.ctrl_thr,
+                         "\"ctrl_thr\":%u,\"plain\":%u,\"ctrl\":%llu,\"m\":[%.17g,%.17g,%.17g,%.17g,%.17g,%.17g,%.17g,%.17g]}",
+                         op.kind, op.target, op.rmask, op.tbit, op.slot, op.en, op.fl, op.ctrl_thr, op.plain,
                          (unsigned long long)op.ctrl, op.m[0], op.m[1], op.m[2], op.m[3], op.m[4], op.m[5], op.m[6],
                          op.m[7]);
                 o += buf;

@@ -82,7 +82,10 @@ constexpr uint32_t kNoReg = 0xFFFFFFFFu;
 //  * en: the physical registers whose register-bit controls hold;
 //  * tbit: the thread bit of a diagonal/IPZ target that is not a register
 //    bit (kNoReg otherwise);
-//  * ctrl_thr: controls that are thread bits (mask over the thread id).
+//  * ctrl_thr: controls that are thread bits (mask over the thread id);
+//  * plain: a GEN op acting on every register (en = all) whose first pair
+//    members (registers with the partner mask's highest bit clear) all hold
+//    target bit 0, so the matrix applies without per-pair selects.
 struct alignas(16) DevOp {
     uint32_t kind;
     uint32_t target;
@@ -93,7 +96,8 @@ struct alignas(16) DevOp {
     uint32_t fl;
     uint32_t ctrl_thr;
     uint64_t ctrl;     // every bit of ctrl must be 1 for the op to act
-    uint64_t pad1;
+    uint32_t plain;
+    uint32_t pad1;
     double m[8];       // re/im of m00, m01, m10, m11
 };
 static_assert(sizeof(DevOp) == 112, "DevOp layout");

@@ -18,11 +18,17 @@ namespace dmv {
 template <class R, int RLOG, int TLOG, bool NT>
 __global__ void __launch_bounds__(1 << TLOG)
 k_tile(typename Cplx<R>::T *__restrict__ state, const DevPass *__restrict__ passes,
-       const DevPhase *__restrict__ phases, const DevOp *__restrict__ ops, uint32_t pass_idx)
+       const DevPhase *__restrict__ phases, const DevOp *__restrict__ ops, uint32_t pass_idx, uint32_t xcd_log2)
 {
     using V = typename Cplx<R>::T;
     __shared__ V lds[1 << (RLOG + TLOG)];
-    tile_body<R, RLOG, TLOG, NT>(state, passes + pass_idx, phases, ops, blockIdx.x, lds);
+    // Workgroups are dealt to the XCDs round-robin.  With xcd_log2 = x > 0
+    // (grid a multiple of 2^x), XCD j takes the j-th contiguous 2^-x share of
+    // the tiles, so the tiles one XCD runs at a time are neighbours in memory.
+    const uint32_t b = blockIdx.x;
+    const uint64_t tile = xcd_log2 ? (uint64_t)(b & ((1u << xcd_log2) - 1)) * (gridDim.x >> xcd_log2) + (b >> xcd_log2)
+                                   : (uint64_t)b;
+    tile_body<R, RLOG, TLOG, NT>(state, passes + pass_idx, phases, ops, tile, lds);
 }
 
 // Persistent form: gridDim.x workgroups stride over the tiles with the next
@@ -76,12 +82,14 @@ static int run_tile(bool nt, void *state, int n, const DevPass *passes, const De
                                (V *)state, passes, phases, ops, pass_idx, tiles);
         return check_launch("pipelined tile pass");
     }
+    const int64_t xl = tune_value("tile_xcd_log2", 3);  // 8 XCDs per MI355X
+    const uint32_t xcd_log2 = (xl > 0 && xl <= 4 && tiles >= (uint64_t(1) << xl)) ? (uint32_t)xl : 0u;
     if (nt)
         hipLaunchKernelGGL((k_tile<R, RLOG, TLOG, true>), dim3((uint32_t)tiles), dim3(1 << TLOG), 0, stream(),
-                           (V *)state, passes, phases, ops, pass_idx);
+                           (V *)state, passes, phases, ops, pass_idx, xcd_log2);
     else
         hipLaunchKernelGGL((k_tile<R, RLOG, TLOG, false>), dim3((uint32_t)tiles), dim3(1 << TLOG), 0, stream(),
-                           (V *)state, passes, phases, ops, pass_idx);
+                           (V *)state, passes, phases, ops, pass_idx, xcd_log2);
     return check_launch("tile pass");
 }
 
@@ -95,6 +103,7 @@ int launch_tile(int dtype, int rlog, int tlog, bool nt, void *state, int n, cons
         if (tlog == 8 && rlog == 4) return run_tile<double, 4, 8>(nt, state, n, passes, phases, ops, pass_idx);
         if (tlog == 8 && rlog == 3) return run_tile<double, 3, 8>(nt, state, n, passes, phases, ops, pass_idx);
         if (tlog == 8 && rlog == 2) return run_tile<double, 2, 8>(nt, state, n, passes, phases, ops, pass_idx);
+        if (tlog == 6 && rlog == 5) return run_tile<double, 5, 6>(nt, state, n, passes, phases, ops, pass_idx);
         if (tlog == 6 && rlog == 4) return run_tile<double, 4, 6>(nt, state, n, passes, phases, ops, pass_idx);
         if (tlog == 6 && rlog == 3) return run_tile<double, 3, 6>(nt, state, n, passes, phases, ops, pass_idx);
     } else {

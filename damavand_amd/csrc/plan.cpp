@@ -323,6 +323,20 @@ static uint32_t pad_regs(uint32_t rs, const PlanParams &pp)
     return rs;
 }
 
+// Registers p with the highest set bit of partner mask m clear: the first
+// member of each pair (p, p ^ m) as the kernels enumerate pairs.
+static uint32_t first_members(uint32_t m, int nr)
+{
+    uint32_t hb = 1;
+    while ((hb << 1) <= m)
+        hb <<= 1;
+    uint32_t f = 0;
+    for (int p = 0; p < nr; p++)
+        if (!(p & hb))
+            f |= 1u << p;
+    return f;
+}
+
 static int make_pass(uint64_t Q, const std::vector<HostOp> &taken, const PlanParams &pp, Plan *plan)
 {
     DevPass P;
@@ -484,12 +498,13 @@ static int make_pass(uint64_t Q, const std::vector<HostOp> &taken, const PlanPar
                 op.rmask = 0;
                 op.fl = 0;
             }
+            if (h.kind == OP_GEN && J != kNoReg)
+                op.plain = (op.en == all_regs && (op.fl & first_members(op.rmask, NR)) == 0) ? 1u : 0u;
             if (op_is_ip(h.kind) && h.slot + 1 > plan->n_slots)
                 plan->n_slots = h.slot + 1;
             plan->ops.push_back(op);
             plan->op_pass.push_back(pass_index);
         }
-        (void)all_regs;
         D.op_end = (uint32_t)plan->ops.size();
         plan->phases.push_back(D);
         prev = D.lay;
@@ -677,6 +692,10 @@ int validate_plan(const Plan &plan, const PlanParams &pp)
                 }
                 if ((op.en & ~regs) || (op.fl & ~regs) || (op.ctrl_thr >> pp.tlog) != 0) {
                     set_error("internal: op masks out of range");
+                    return DMV_ERR_STATE;
+                }
+                if (op.plain && (op.kind != OP_GEN || op.en != regs || (op.fl & first_members(op.rmask, NR)) != 0)) {
+                    set_error("internal: plain flag on an op that needs selects");
                     return DMV_ERR_STATE;
                 }
                 const bool target_reg = register_bit(D.lay, pp, op.target) != kNoReg;

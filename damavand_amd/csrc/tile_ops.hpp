@@ -65,37 +65,21 @@ template <int M, class V, class R, int NR>
 __device__ __forceinline__ void op_gen_m(V (&v)[NR], const Mat<R> &A, uint32_t en, uint32_t fl)
 {
     constexpr int HB = highest_bit(M);
-    const uint32_t all = NR >= 32 ? 0xFFFFFFFFu : ((1u << NR) - 1);
-    if (en == all) {
 #pragma unroll
-        for (int p = 0; p < NR; p++) {
-            if (p & HB)
-                continue;
-            const bool f = (fl >> p) & 1u;
-            Mat<R> B;
-            B.m00r = f ? A.m11r : A.m00r; B.m00i = f ? A.m11i : A.m00i;
-            B.m01r = f ? A.m10r : A.m01r; B.m01i = f ? A.m10i : A.m01i;
-            B.m10r = f ? A.m01r : A.m10r; B.m10i = f ? A.m01i : A.m10i;
-            B.m11r = f ? A.m00r : A.m11r; B.m11i = f ? A.m00i : A.m11i;
-            apply2(v[p], v[p ^ M], B);
-        }
-    } else {
-#pragma unroll
-        for (int p = 0; p < NR; p++) {
-            if (p & HB)
-                continue;
-            const bool f = (fl >> p) & 1u;
-            Mat<R> B;
-            B.m00r = f ? A.m11r : A.m00r; B.m00i = f ? A.m11i : A.m00i;
-            B.m01r = f ? A.m10r : A.m01r; B.m01i = f ? A.m10i : A.m01i;
-            B.m10r = f ? A.m01r : A.m10r; B.m10i = f ? A.m01i : A.m10i;
-            B.m11r = f ? A.m00r : A.m11r; B.m11i = f ? A.m00i : A.m11i;
-            V x = v[p], y = v[p ^ M];
-            apply2(x, y, B);
-            const bool ok = (en >> p) & 1u;
-            v[p] = ok ? x : v[p];
-            v[p ^ M] = ok ? y : v[p ^ M];
-        }
+    for (int p = 0; p < NR; p++) {
+        if (p & HB)
+            continue;
+        const bool f = (fl >> p) & 1u;
+        Mat<R> B;
+        B.m00r = f ? A.m11r : A.m00r; B.m00i = f ? A.m11i : A.m00i;
+        B.m01r = f ? A.m10r : A.m01r; B.m01i = f ? A.m10i : A.m01i;
+        B.m10r = f ? A.m01r : A.m10r; B.m10i = f ? A.m01i : A.m10i;
+        B.m11r = f ? A.m00r : A.m11r; B.m11i = f ? A.m00i : A.m11i;
+        V x = v[p], y = v[p ^ M];
+        apply2(x, y, B);
+        const bool ok = (en >> p) & 1u;
+        v[p] = ok ? x : v[p];
+        v[p ^ M] = ok ? y : v[p ^ M];
     }
 }
 
@@ -180,6 +164,35 @@ __device__ __forceinline__ bool thread_controls_hold(const DevOp *op, uint32_t t
     return (tid & ct) == ct;
 }
 
+// GEN with the plain flag: every pair (p, p ^ M) in order, no selects.
+template <int M, class V, class R, int NR>
+__device__ __forceinline__ void op_gen_plain_m(V (&v)[NR], const Mat<R> &A)
+{
+    constexpr int HB = highest_bit(M);
+#pragma unroll
+    for (int p = 0; p < NR; p++) {
+        if (p & HB)
+            continue;
+        apply2(v[p], v[p ^ M], A);
+    }
+}
+
+template <class V, class R, int NR>
+__device__ __forceinline__ void op_gen_plain(V (&v)[NR], const DevOp *op)
+{
+    const Mat<R> A = load_mat<R>(op->m);
+    switch (op->rmask) {
+#define DMV_PLAIN_CASE(M)                                                                          \
+    case M:                                                                                        \
+        if constexpr (M < NR)                                                                      \
+            op_gen_plain_m<M>(v, A);                                                               \
+        break;
+        DMV_MASK_CASES(DMV_PLAIN_CASE)
+#undef DMV_PLAIN_CASE
+    default: break;
+    }
+}
+
 template <class V, class R, int RLOG>
 __device__ __forceinline__ void run_ops(V (&v)[1 << RLOG], const DevOp *__restrict__ ops, uint32_t ob, uint32_t oe,
                                         uint32_t tid)
@@ -188,7 +201,9 @@ __device__ __forceinline__ void run_ops(V (&v)[1 << RLOG], const DevOp *__restri
         const DevOp *op = ops + o;
         if (!thread_controls_hold(op, tid))
             continue;
-        if (op->kind == OP_DIAG)
+        if (op->plain)
+            op_gen_plain<V, R>(v, op);
+        else if (op->kind == OP_DIAG)
             op_diag<V, R>(v, op, tid);
         else
             op_pair<V, R>(v, op);

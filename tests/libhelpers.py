@@ -24,6 +24,7 @@ class NativeCircuit:
 
     def __init__(self, n, gates):
         self.n = n
+        self.gates = list(gates)
         h = ctypes.c_void_p()
         N.call("dmv_circuit_create", n, ctypes.byref(h))
         self.h = h

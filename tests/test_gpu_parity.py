@@ -100,21 +100,32 @@ def test_pipelined_tile_passes(native, n, grid, nt):
     assert err <= TOL
 
 
-@pytest.mark.parametrize("n,k,tlog,gen", [(20, 13, 9, 8), (19, 12, 9, 4), (20, 11, 8, 40), (22, 12, 8, 2)])
-def test_tile_shapes(native, n, k, tlog, gen):
-    """Every tile shape the tuner may pick (512- and 256-thread workgroups,
-    11- to 13-qubit tiles, different compute budgets per pass)."""
-    say(f"case tile shape n={n} k={k} tlog={tlog} gen={gen}")
+TILE_CASES = [
+    (20, 13, 9, 8, {}), (19, 12, 9, 4, {}), (20, 11, 8, 40, {}), (22, 12, 8, 2, {}), (20, 11, 6, 8, {}),
+    (20, 10, 6, 16, {}), (21, 12, 8, 8, {"tile_xcd_log2": 0}), (20, 12, 8, 8, {"coalesce_qubits": 5}),
+    (20, 12, 8, 8, {"register_renaming": 0}),
+]
+
+
+@pytest.mark.parametrize("n,k,tlog,gen,extra", TILE_CASES)
+def test_tile_shapes(native, n, k, tlog, gen, extra):
+    """Every tile shape and launch option the tuner may pick (64- to
+    512-thread workgroups, 10- to 13-qubit tiles, compute budgets per pass,
+    XCD-contiguous tile order, wider coalescing runs, renaming off)."""
+    say(f"case tile shape n={n} k={k} tlog={tlog} gen={gen} {extra}")
     rng = np.random.default_rng(400 + n + k)
     gates = random_gates(rng, n, 300)
     c = build_circuit(n, gates)
-    for key, v in (("tile_qubits", k), ("tile_threads_log2", tlog), ("max_gen_per_pass", gen)):
+    knobs = dict(tile_qubits=k, tile_threads_log2=tlog, max_gen_per_pass=gen, **extra)
+    defaults = dict(tile_qubits=0, tile_threads_log2=0, max_gen_per_pass=8, tile_xcd_log2=3, coalesce_qubits=0,
+                    register_renaming=1)
+    for key, v in knobs.items():
         N.tune_set(key, v)
     try:
         c.forward()
     finally:
-        for key in ("tile_qubits", "tile_threads_log2", "max_gen_per_pass"):
-            N.tune_set(key, 0 if key != "max_gen_per_pass" else 8)
+        for key in knobs:
+            N.tune_set(key, defaults[key])
     err = np.max(np.abs(c.retrieve_amplitudes_on_host() - oracle.run_gates(n, gates).amp))
     say(f"  passes={c.plan_stats()[0]} max-abs error {err:.3e}")
     assert err <= TOL

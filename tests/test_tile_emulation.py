@@ -35,7 +35,7 @@ def zero_state(n):
 
 @pytest.mark.parametrize("n,k,tlog,gen,seed", [
     (13, 12, 8, 8, 1), (14, 12, 8, 4, 2), (15, 11, 8, 8, 3), (14, 9, 6, 8, 4), (15, 13, 9, 8, 5),
-    (16, 12, 9, 40, 6), (14, 10, 8, 2, 7), (13, 12, 8, 100, 8)])
+    (16, 12, 9, 40, 6), (14, 10, 8, 2, 7), (13, 12, 8, 100, 8), (14, 11, 6, 8, 9), (13, 10, 6, 16, 10)])
 def test_forward_plan_emulation(n, k, tlog, gen, seed):
     rng = np.random.default_rng(seed)
     gates = random_gates(rng, n, 250)
@@ -64,6 +64,17 @@ def test_complex64_plan_emulation():
     np.testing.assert_allclose(got, oracle.run_gates(n, gates).amp, atol=1e-12)
 
 
+def test_plain_flag_marks_select_free_ops():
+    """Most GEN ops of a layered circuit need no per-pair selects, and the flag says so exactly."""
+    n = 16
+    nc = NativeCircuit(n, wl.layered_circuit(n, 3, 5))
+    plan = plan_json(nc.h, tile_qubits=12, tlog=8, max_gen=8)
+    gens = [op for P in plan["passes"] for ph in P["phases"] for op in ph["ops"] if op["kind"] == 0]
+    assert sum(op["plain"] for op in gens) >= 0.9 * len(gens)
+    got = TileEmulator(plan).forward(zero_state(n))
+    np.testing.assert_allclose(got, oracle.run_gates(n, nc.gates).amp, atol=1e-12)
+
+
 def test_renaming_removes_register_cnots():
     """CNOT ladders among register qubits emit no device op (only the map changes)."""
     n = 14
@@ -85,8 +96,9 @@ def test_planted_mask_mistake_is_caught():
     for P in plan["passes"]:
         for ph in P["phases"]:
             for op in ph["ops"]:
-                if op["kind"] == 0 and op["fl"]:
+                if op["kind"] == 0:
                     op["fl"] ^= 1  # flip the member assignment of one pair
+                    op["plain"] = 0
                     break
     got = TileEmulator(plan).forward(zero_state(n))
     assert np.max(np.abs(got - oracle.run_gates(n, gates).amp)) > 1e-6

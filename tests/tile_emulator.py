@@ -141,6 +141,12 @@ class TileEmulator:
             return
         hb = highest_bit(rm)
         assert 0 < rm < self.NR
+        if op["plain"]:
+            # the kernel applies the matrix as is to every pair (p, p ^ rmask), p the first member;
+            # the host claims that this is what the masks say
+            first = sum(1 << q for q in range(self.NR) if not q & hb)
+            assert kind == OP_GEN and en == (1 << self.NR) - 1 and (fl & first) == 0, "plain flag inconsistent"
+            en, fl = (1 << self.NR) - 1, 0
         for p in range(self.NR):
             if p & hb:
                 continue
