@@ -64,7 +64,7 @@ int dmv_state_bytes(int num_qubits, int dtype, uint64_t *bytes)
 {
     DMV_CHECK_ARG(bytes != nullptr, "null pointer");
     DMV_CHECK_ARG(num_qubits >= 1 && num_qubits <= kMaxQubits, "num_qubits out of range");
-    const uint64_t amp = (uint64_t(1) << num_qubits) * (uint64_t)amp_bytes(dtype);
+    const uint64_t amp = state_elems(num_qubits, state_layout(num_qubits, dtype == DMV_C64)) * (uint64_t)amp_bytes(dtype);
     const int B = num_qubits < 12 ? num_qubits : 12;
     const uint64_t nblk = uint64_t(1) << (num_qubits - B);
     *bytes = amp + (nblk * 13 + 64) * 8 * 2 + 64 * 8;
@@ -150,7 +150,7 @@ int dmv_state_copy(dmv_state *dst, const dmv_state *src)
     DMV_CHECK_ARG(dst->n == src->n && dst->dtype == src->dtype, "states differ in size or dtype");
     DMV_TRY(ensure_device());
     prof_begin(DMV_KC_COPY);
-    DMV_TRY(launch_copy(dst->amp, src->amp, dst->bytes));
+    DMV_TRY(launch_copy(dst->amp, src->amp, dst->alloc_bytes));
     prof_end(DMV_KC_COPY, 2.0 * (double)dst->bytes);
     return DMV_OK;
 }
@@ -161,7 +161,18 @@ int dmv_state_device_ptr(dmv_state *s, void **ptr, uint64_t *bytes)
     if (ptr)
         *ptr = s->amp;
     if (bytes)
-        *bytes = s->bytes;
+        *bytes = s->alloc_bytes;
+    return DMV_OK;
+}
+
+int dmv_state_layout(const dmv_state *s, int *shift, int *gap)
+{
+    CHECK_STATE(s);
+    const StateLayout L = state_layout(s->n, s->dtype == DMV_C64);
+    if (shift)
+        *shift = (int)L.shift;
+    if (gap)
+        *gap = (int)L.gap;
     return DMV_OK;
 }
 
@@ -621,6 +632,19 @@ int dmv_circuit_plan_ops(dmv_circuit *c, int dtype, int tile_qubits, int *kind, 
     return DMV_OK;
 }
 
+static void json_u64s(std::string &o, const char *key, const uint64_t *a, int n)
+{
+    o += "\"";
+    o += key;
+    o += "\":[";
+    for (int i = 0; i < n; i++) {
+        if (i)
+            o += ",";
+        o += std::to_string(a[i]);
+    }
+    o += "]";
+}
+
 static void json_u32s(std::string &o, const char *key, const uint32_t *a, int n)
 {
     o += "\"";
@@ -664,7 +688,8 @@ int dmv_circuit_plan_json(dmv_circuit *c, int dtype, int adjoint, int tile_qubit
     DMV_TRY(build_plan(ops, pp, &plan));
     std::string o = "{";
     o += "\"n\":" + std::to_string(pp.n) + ",\"k\":" + std::to_string(pp.k) + ",\"rlog\":" + std::to_string(pp.rlog) +
-         ",\"tlog\":" + std::to_string(pp.tlog) + ",\"n_slots\":" + std::to_string(plan.n_slots) + ",\"passes\":[";
+         ",\"tlog\":" + std::to_string(pp.tlog) + ",\"n_slots\":" + std::to_string(plan.n_slots) +
+         ",\"layout\":[" + std::to_string(pp.layout.shift) + "," + std::to_string(pp.layout.gap) + "],\"passes\":[";
     for (size_t i = 0; i < plan.passes.size(); i++) {
         const DevPass &P = plan.passes[i];
         if (i)
@@ -672,14 +697,9 @@ int dmv_circuit_plan_json(dmv_circuit *c, int dtype, int adjoint, int tile_qubit
         o += "{";
         json_u32s(o, "qpos", P.qpos, pp.k);
         o += ",";
-        std::vector<uint32_t> gl(1 << pp.rlog), gs(1 << pp.rlog);
-        for (int r = 0; r < (1 << pp.rlog); r++) {
-            gl[r] = (uint32_t)P.greg_ld[r];  // < 2^32 for the sizes this is used at
-            gs[r] = (uint32_t)P.greg_st[r];
-        }
-        json_u32s(o, "greg_ld", gl.data(), 1 << pp.rlog);
+        json_u64s(o, "greg_ld", P.greg_ld, 1 << pp.rlog);
         o += ",";
-        json_u32s(o, "greg_st", gs.data(), 1 << pp.rlog);
+        json_u64s(o, "greg_st", P.greg_st, 1 << pp.rlog);
         o += ",";
         json_u32s(o, "gthr_ld", P.gthr_ld, pp.tlog);
         o += ",";

@@ -27,37 +27,39 @@ __device__ __forceinline__ uint64_t expand(uint64_t p, const Positions &P)
 template <class R, int PPT, int MODE, bool NT>
 __global__ void __launch_bounds__(kGateThreads)
 k_gate(typename Cplx<R>::T *__restrict__ state, uint64_t n_items, uint32_t t, Positions P, uint64_t ormask,
-       Mat<R> M)
+       Mat<R> M, StateLayout L)
 {
     using V = typename Cplx<R>::T;
     const uint64_t base = (uint64_t)blockIdx.x * (kGateThreads * PPT) + threadIdx.x;
-    uint64_t idx[PPT];
+    uint64_t idx[PPT], e[PPT];
     bool ok[PPT];
 #pragma unroll
     for (int u = 0; u < PPT; u++) {
         const uint64_t p = base + (uint64_t)u * kGateThreads;
         ok[u] = p < n_items;
         idx[u] = expand(p, P) | ormask;
+        e[u] = elem(idx[u], L);
     }
-    const uint64_t tb = uint64_t(1) << t;
+    // idx has bit t clear, so the partner's element is e + elem(2^t)
+    const uint64_t tb = elem(uint64_t(1) << t, L);
     if constexpr (MODE == GM_GEN || MODE == GM_SWAP) {
         V a[PPT], b[PPT];
 #pragma unroll
         for (int u = 0; u < PPT; u++)
             if (ok[u]) {
-                a[u] = ld<NT>(state + idx[u]);
-                b[u] = ld<NT>(state + (idx[u] | tb));
+                a[u] = ld<NT>(state + e[u]);
+                b[u] = ld<NT>(state + (e[u] + tb));
             }
 #pragma unroll
         for (int u = 0; u < PPT; u++)
             if (ok[u]) {
                 if constexpr (MODE == GM_GEN) {
                     apply2(a[u], b[u], M);
-                    st<NT>(state + idx[u], a[u]);
-                    st<NT>(state + (idx[u] | tb), b[u]);
+                    st<NT>(state + e[u], a[u]);
+                    st<NT>(state + (e[u] + tb), b[u]);
                 } else {
-                    st<NT>(state + idx[u], b[u]);
-                    st<NT>(state + (idx[u] | tb), a[u]);
+                    st<NT>(state + e[u], b[u]);
+                    st<NT>(state + (e[u] + tb), a[u]);
                 }
             }
     } else {
@@ -65,7 +67,7 @@ k_gate(typename Cplx<R>::T *__restrict__ state, uint64_t n_items, uint32_t t, Po
 #pragma unroll
         for (int u = 0; u < PPT; u++)
             if (ok[u])
-                a[u] = ld<NT>(state + idx[u]);
+                a[u] = ld<NT>(state + e[u]);
 #pragma unroll
         for (int u = 0; u < PPT; u++)
             if (ok[u]) {
@@ -75,14 +77,14 @@ k_gate(typename Cplx<R>::T *__restrict__ state, uint64_t n_items, uint32_t t, Po
                 } else {
                     a[u] = cmul(a[u], M.m11r, M.m11i);
                 }
-                st<NT>(state + idx[u], a[u]);
+                st<NT>(state + e[u], a[u]);
             }
     }
 }
 
 template <class R, int PPT, int MODE>
 static int run_gate(bool nt, void *state, uint64_t items, uint32_t t, const Positions &P, uint64_t ormask,
-                    const Mat<R> &M)
+                    const Mat<R> &M, StateLayout L)
 {
     using V = typename Cplx<R>::T;
     const uint64_t per_block = (uint64_t)kGateThreads * PPT;
@@ -90,33 +92,33 @@ static int run_gate(bool nt, void *state, uint64_t items, uint32_t t, const Posi
     DMV_CHECK_ARG(blocks > 0 && blocks <= 0xFFFFFFFFull, "gate grid out of range");
     if (nt)
         hipLaunchKernelGGL((k_gate<R, PPT, MODE, true>), dim3((uint32_t)blocks), dim3(kGateThreads), 0, stream(),
-                           (V *)state, items, t, P, ormask, M);
+                           (V *)state, items, t, P, ormask, M, L);
     else
         hipLaunchKernelGGL((k_gate<R, PPT, MODE, false>), dim3((uint32_t)blocks), dim3(kGateThreads), 0,
-                           stream(), (V *)state, items, t, P, ormask, M);
+                           stream(), (V *)state, items, t, P, ormask, M, L);
     return check_launch("gate kernel");
 }
 
 template <class R, int MODE>
 static int run_gate_ppt(int ppt, bool nt, void *state, uint64_t items, uint32_t t, const Positions &P,
-                        uint64_t ormask, const Mat<R> &M)
+                        uint64_t ormask, const Mat<R> &M, StateLayout L)
 {
     if (ppt >= 4)
-        return run_gate<R, 4, MODE>(nt, state, items, t, P, ormask, M);
+        return run_gate<R, 4, MODE>(nt, state, items, t, P, ormask, M, L);
     if (ppt == 2)
-        return run_gate<R, 2, MODE>(nt, state, items, t, P, ormask, M);
-    return run_gate<R, 1, MODE>(nt, state, items, t, P, ormask, M);
+        return run_gate<R, 2, MODE>(nt, state, items, t, P, ormask, M, L);
+    return run_gate<R, 1, MODE>(nt, state, items, t, P, ormask, M, L);
 }
 
 template <class R>
 static int gate_dispatch(int mode, int ppt, bool nt, void *state, uint64_t items, uint32_t t,
-                         const Positions &P, uint64_t ormask, const Mat<R> &M)
+                         const Positions &P, uint64_t ormask, const Mat<R> &M, StateLayout L)
 {
     switch (mode) {
-    case GM_GEN: return run_gate_ppt<R, GM_GEN>(ppt, nt, state, items, t, P, ormask, M);
-    case GM_SWAP: return run_gate_ppt<R, GM_SWAP>(ppt, nt, state, items, t, P, ormask, M);
-    case GM_DIAG: return run_gate_ppt<R, GM_DIAG>(ppt, nt, state, items, t, P, ormask, M);
-    default: return run_gate_ppt<R, GM_PHASE>(ppt, nt, state, items, t, P, ormask, M);
+    case GM_GEN: return run_gate_ppt<R, GM_GEN>(ppt, nt, state, items, t, P, ormask, M, L);
+    case GM_SWAP: return run_gate_ppt<R, GM_SWAP>(ppt, nt, state, items, t, P, ormask, M, L);
+    case GM_DIAG: return run_gate_ppt<R, GM_DIAG>(ppt, nt, state, items, t, P, ormask, M, L);
+    default: return run_gate_ppt<R, GM_PHASE>(ppt, nt, state, items, t, P, ormask, M, L);
     }
 }
 
@@ -153,8 +155,10 @@ int launch_gate(int dtype, void *state, int n, const DevOp &op, int ppt, bool nt
     // one work item per changed pair (GEN/SWAP) or amplitude (DIAG/PHASE)
     const uint64_t items = (uint64_t(1) << n) >> P.count;
     if (dtype == DMV_C64)
-        return gate_dispatch<float>(mode, ppt, nt, state, items, op.target, P, ormask, load_mat<float>(op.m));
-    return gate_dispatch<double>(mode, ppt, nt, state, items, op.target, P, ormask, load_mat<double>(op.m));
+        return gate_dispatch<float>(mode, ppt, nt, state, items, op.target, P, ormask, load_mat<float>(op.m),
+                                    state_layout(n, true));
+    return gate_dispatch<double>(mode, ppt, nt, state, items, op.target, P, ormask, load_mat<double>(op.m),
+                                 state_layout(n, false));
 }
 
 }  // namespace dmv

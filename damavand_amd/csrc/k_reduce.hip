@@ -19,14 +19,14 @@ static uint32_t grid_for(uint64_t items, uint64_t per_block, uint32_t cap = 1u <
 
 // ------------------------------------------------------------- elementwise --
 template <class V>
-__global__ void k_init_basis(V *__restrict__ s, uint64_t n_amp, uint64_t basis)
+__global__ void k_init_basis(V *__restrict__ s, uint64_t n_amp, uint64_t basis, StateLayout L)
 {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_amp;
          i += (uint64_t)gridDim.x * blockDim.x) {
         V v;
         v.x = i == basis ? 1 : 0;
         v.y = 0;
-        s[i] = v;
+        s[elem(i, L)] = v;
     }
 }
 
@@ -35,10 +35,11 @@ int launch_init_basis(int dtype, void *state, int n, uint64_t basis)
     const uint64_t n_amp = uint64_t(1) << n;
     DMV_CHECK_ARG(basis < n_amp || basis == ~uint64_t(0), "basis index out of range");
     const uint32_t g = grid_for(n_amp, RT * 8);
+    const StateLayout L = state_layout(n, dtype == DMV_C64);
     if (dtype == DMV_C64)
-        hipLaunchKernelGGL(k_init_basis<c64>, dim3(g), dim3(RT), 0, stream(), (c64 *)state, n_amp, basis);
+        hipLaunchKernelGGL(k_init_basis<c64>, dim3(g), dim3(RT), 0, stream(), (c64 *)state, n_amp, basis, L);
     else
-        hipLaunchKernelGGL(k_init_basis<c128>, dim3(g), dim3(RT), 0, stream(), (c128 *)state, n_amp, basis);
+        hipLaunchKernelGGL(k_init_basis<c128>, dim3(g), dim3(RT), 0, stream(), (c128 *)state, n_amp, basis, L);
     return check_launch("init");
 }
 
@@ -68,75 +69,79 @@ int launch_copy(void *dst, const void *src, uint64_t bytes)
 }
 
 template <class V>
-__global__ void k_split(const V *__restrict__ s, uint64_t begin, uint64_t count, double *re, double *im)
+__global__ void k_split(const V *__restrict__ s, uint64_t begin, uint64_t count, double *re, double *im,
+                        StateLayout L)
 {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (uint64_t)gridDim.x * blockDim.x) {
-        const V v = s[begin + i];
+        const V v = s[elem(begin + i, L)];
         re[i] = v.x;
         im[i] = v.y;
     }
 }
 
 template <class V>
-__global__ void k_join(V *__restrict__ s, uint64_t begin, uint64_t count, const double *re, const double *im)
+__global__ void k_join(V *__restrict__ s, uint64_t begin, uint64_t count, const double *re, const double *im,
+                       StateLayout L)
 {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (uint64_t)gridDim.x * blockDim.x) {
         V v;
         v.x = re[i];
         v.y = im[i];
-        s[begin + i] = v;
+        s[elem(begin + i, L)] = v;
     }
 }
 
 template <class V>
-__global__ void k_probs(const V *__restrict__ s, uint64_t begin, uint64_t count, double *out)
+__global__ void k_probs(const V *__restrict__ s, uint64_t begin, uint64_t count, double *out, StateLayout L)
 {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (uint64_t)gridDim.x * blockDim.x) {
-        const V v = s[begin + i];
+        const V v = s[elem(begin + i, L)];
         out[i] = (double)v.x * v.x + (double)v.y * v.y;
     }
 }
 
-int launch_split(int dtype, const void *state, uint64_t begin, uint64_t count, double *re, double *im)
+int launch_split(int dtype, const void *state, StateLayout L, uint64_t begin, uint64_t count, double *re, double *im)
 {
     const uint32_t g = grid_for(count, RT * 4);
     if (dtype == DMV_C64)
-        hipLaunchKernelGGL(k_split<c64>, dim3(g), dim3(RT), 0, stream(), (const c64 *)state, begin, count, re, im);
+        hipLaunchKernelGGL(k_split<c64>, dim3(g), dim3(RT), 0, stream(), (const c64 *)state, begin, count, re, im, L);
     else
-        hipLaunchKernelGGL(k_split<c128>, dim3(g), dim3(RT), 0, stream(), (const c128 *)state, begin, count, re, im);
+        hipLaunchKernelGGL(k_split<c128>, dim3(g), dim3(RT), 0, stream(), (const c128 *)state, begin, count, re, im, L);
     return check_launch("split");
 }
 
-int launch_join(int dtype, void *state, uint64_t begin, uint64_t count, const double *re, const double *im)
+int launch_join(int dtype, void *state, StateLayout L, uint64_t begin, uint64_t count, const double *re,
+                const double *im)
 {
     const uint32_t g = grid_for(count, RT * 4);
     if (dtype == DMV_C64)
-        hipLaunchKernelGGL(k_join<c64>, dim3(g), dim3(RT), 0, stream(), (c64 *)state, begin, count, re, im);
+        hipLaunchKernelGGL(k_join<c64>, dim3(g), dim3(RT), 0, stream(), (c64 *)state, begin, count, re, im, L);
     else
-        hipLaunchKernelGGL(k_join<c128>, dim3(g), dim3(RT), 0, stream(), (c128 *)state, begin, count, re, im);
+        hipLaunchKernelGGL(k_join<c128>, dim3(g), dim3(RT), 0, stream(), (c128 *)state, begin, count, re, im, L);
     return check_launch("join");
 }
 
-int launch_probs(int dtype, const void *state, uint64_t begin, uint64_t count, double *out)
+int launch_probs(int dtype, const void *state, StateLayout L, uint64_t begin, uint64_t count, double *out)
 {
     const uint32_t g = grid_for(count, RT * 4);
     if (dtype == DMV_C64)
-        hipLaunchKernelGGL(k_probs<c64>, dim3(g), dim3(RT), 0, stream(), (const c64 *)state, begin, count, out);
+        hipLaunchKernelGGL(k_probs<c64>, dim3(g), dim3(RT), 0, stream(), (const c64 *)state, begin, count, out, L);
     else
-        hipLaunchKernelGGL(k_probs<c128>, dim3(g), dim3(RT), 0, stream(), (const c128 *)state, begin, count, out);
+        hipLaunchKernelGGL(k_probs<c128>, dim3(g), dim3(RT), 0, stream(), (const c128 *)state, begin, count, out, L);
     return check_launch("probabilities");
 }
 
 // --------------------------------------------------------------- reductions --
 template <class V>
-__global__ void __launch_bounds__(RT) k_block_norms(const V *__restrict__ s, uint32_t bb, double *partials)
+__global__ void __launch_bounds__(RT) k_block_norms(const V *__restrict__ s, uint32_t bb, double *partials,
+                                                   StateLayout L)
 {
     __shared__ double red[RT / 64];
-    const uint64_t base = (uint64_t)blockIdx.x << bb;
+    const V *blk = s + elem((uint64_t)blockIdx.x << bb, L);  // a block never spans a gap
     const uint32_t span = 1u << bb;
     double acc = 0.0;
     for (uint32_t i = threadIdx.x; i < span; i += RT)
-        acc += norm2(s[base + i]);
+        acc += norm2(blk[i]);
     acc = block_sum<RT>(acc, red);
     if (threadIdx.x == 0)
         partials[blockIdx.x] = acc;
@@ -144,14 +149,15 @@ __global__ void __launch_bounds__(RT) k_block_norms(const V *__restrict__ s, uin
 
 int launch_block_norms(int dtype, const void *state, int n, int bb, double *partials)
 {
-    DMV_CHECK_ARG(bb >= 0 && bb <= n && n - bb <= 31, "block bits out of range");
+    const StateLayout L = state_layout(n, dtype == DMV_C64);
+    DMV_CHECK_ARG(bb >= 0 && bb <= n && n - bb <= 31 && (L.gap == 0 || bb <= (int)L.shift), "block bits out of range");
     const uint32_t blocks = 1u << (n - bb);
     if (dtype == DMV_C64)
         hipLaunchKernelGGL(k_block_norms<c64>, dim3(blocks), dim3(RT), 0, stream(), (const c64 *)state, (uint32_t)bb,
-                           partials);
+                           partials, L);
     else
         hipLaunchKernelGGL(k_block_norms<c128>, dim3(blocks), dim3(RT), 0, stream(), (const c128 *)state,
-                           (uint32_t)bb, partials);
+                           (uint32_t)bb, partials, L);
     return check_launch("block norms");
 }
 
@@ -184,17 +190,18 @@ int launch_sum(const double *in, uint64_t count, double *out)
 // <Z_q>: per block of 2^B amplitudes, the total and, for q < B, the mass
 // with bit q set.  Bits >= B are constant over a block.
 template <class V>
-__global__ void __launch_bounds__(RT) k_expect_z_part(const V *__restrict__ s, uint32_t B, double *partials)
+__global__ void __launch_bounds__(RT) k_expect_z_part(const V *__restrict__ s, uint32_t B, double *partials,
+                                                     StateLayout L)
 {
     __shared__ double red[RT / 64];
-    const uint64_t base = (uint64_t)blockIdx.x << B;
+    const V *blk = s + elem((uint64_t)blockIdx.x << B, L);
     const uint32_t span = 1u << B;
     double tot = 0.0, m[kBlockBits];
 #pragma unroll
     for (int q = 0; q < kBlockBits; q++)
         m[q] = 0.0;
     for (uint32_t i = threadIdx.x; i < span; i += RT) {
-        const double p = norm2(s[base + i]);
+        const double p = norm2(blk[i]);
         tot += p;
 #pragma unroll
         for (int q = 0; q < kBlockBits; q++)
@@ -246,12 +253,13 @@ int launch_expect_z(int dtype, const void *state, int n, double *partials, doubl
 {
     const int B = n < kBlockBits ? n : kBlockBits;
     const uint64_t nblk = uint64_t(1) << (n - B);
+    const StateLayout L = state_layout(n, dtype == DMV_C64);
     if (dtype == DMV_C64)
         hipLaunchKernelGGL(k_expect_z_part<c64>, dim3((uint32_t)nblk), dim3(RT), 0, stream(), (const c64 *)state,
-                           (uint32_t)B, partials);
+                           (uint32_t)B, partials, L);
     else
         hipLaunchKernelGGL(k_expect_z_part<c128>, dim3((uint32_t)nblk), dim3(RT), 0, stream(), (const c128 *)state,
-                           (uint32_t)B, partials);
+                           (uint32_t)B, partials, L);
     DMV_TRY(check_launch("<Z> partials"));
     hipLaunchKernelGGL(k_expect_z_final, dim3(1), dim3(RT), 0, stream(), partials, nblk, (uint32_t)B, (uint32_t)n, out);
     return check_launch("<Z> final");
@@ -270,7 +278,7 @@ constexpr int kMaxPairs = 128;
 
 template <class V, bool WRITE>
 __global__ void __launch_bounds__(RT) k_obs(const V *__restrict__ s, V *__restrict__ lam, uint32_t n,
-                                            uint32_t B, ObsTerms T, double *partials)
+                                            uint32_t B, ObsTerms T, double *partials, StateLayout L)
 {
     __shared__ double red[RT / 64];
     __shared__ double sa[kMaxQubits];
@@ -284,6 +292,7 @@ __global__ void __launch_bounds__(RT) k_obs(const V *__restrict__ s, V *__restri
     }
     __syncthreads();
     const uint64_t base = (uint64_t)blockIdx.x << B;
+    const uint64_t e0 = elem(base, L);
     const uint32_t span = 1u << B;
     double acc = 0.0;
     for (uint32_t i = threadIdx.x; i < span; i += RT) {
@@ -293,13 +302,13 @@ __global__ void __launch_bounds__(RT) k_obs(const V *__restrict__ s, V *__restri
             h += ((x >> q) & 1u) ? -sa[q] : sa[q];
         for (int t = 0; t < T.n_pairs; t++)
             h += (__popcll(x & smask[t]) & 1) ? -sb[t] : sb[t];
-        const V v = s[x];
+        const V v = s[e0 + i];
         acc += h * norm2(v);
         if constexpr (WRITE) {
             V w;
             w.x = v.x * h;
             w.y = v.y * h;
-            lam[x] = w;
+            lam[e0 + i] = w;
         }
     }
     acc = block_sum<RT>(acc, red);
@@ -315,12 +324,13 @@ static int run_obs(int dtype, const void *psi, void *lam, int n, const double *a
     const int B = n < kBlockBits ? n : kBlockBits;
     const uint64_t nblk = uint64_t(1) << (n - B);
     ObsTerms T{a, pi, pj, b, n_pairs};
+    const StateLayout L = state_layout(n, dtype == DMV_C64);
     if (dtype == DMV_C64)
         hipLaunchKernelGGL((k_obs<c64, WRITE>), dim3((uint32_t)nblk), dim3(RT), 0, stream(), (const c64 *)psi,
-                           (c64 *)lam, (uint32_t)n, (uint32_t)B, T, partials);
+                           (c64 *)lam, (uint32_t)n, (uint32_t)B, T, partials, L);
     else
         hipLaunchKernelGGL((k_obs<c128, WRITE>), dim3((uint32_t)nblk), dim3(RT), 0, stream(), (const c128 *)psi,
-                           (c128 *)lam, (uint32_t)n, (uint32_t)B, T, partials);
+                           (c128 *)lam, (uint32_t)n, (uint32_t)B, T, partials, L);
     DMV_TRY(check_launch("<H> partials"));
     return launch_sum(partials, nblk, out);
 }
@@ -339,10 +349,10 @@ int launch_apply_obs(int dtype, const void *psi, void *lam, int n, const double 
 
 template <class V>
 __global__ void __launch_bounds__(RT) k_inner(const V *__restrict__ a, const V *__restrict__ b, uint32_t B,
-                                              double *partials)
+                                              double *partials, StateLayout L)
 {
     __shared__ double red[RT / 64];
-    const uint64_t base = (uint64_t)blockIdx.x << B;
+    const uint64_t base = elem((uint64_t)blockIdx.x << B, L);
     const uint32_t span = 1u << B;
     double re = 0.0, im = 0.0;
     for (uint32_t i = threadIdx.x; i < span; i += RT) {
@@ -362,12 +372,13 @@ int launch_inner(int dtype, const void *a, const void *b, int n, double *partial
 {
     const int B = n < kBlockBits ? n : kBlockBits;
     const uint64_t nblk = uint64_t(1) << (n - B);
+    const StateLayout L = state_layout(n, dtype == DMV_C64);
     if (dtype == DMV_C64)
         hipLaunchKernelGGL(k_inner<c64>, dim3((uint32_t)nblk), dim3(RT), 0, stream(), (const c64 *)a, (const c64 *)b,
-                           (uint32_t)B, partials);
+                           (uint32_t)B, partials, L);
     else
         hipLaunchKernelGGL(k_inner<c128>, dim3((uint32_t)nblk), dim3(RT), 0, stream(), (const c128 *)a,
-                           (const c128 *)b, (uint32_t)B, partials);
+                           (const c128 *)b, (uint32_t)B, partials, L);
     DMV_TRY(check_launch("inner product partials"));
     hipLaunchKernelGGL(k_sum, dim3(1), dim3(RT), 0, stream(), partials, nblk, 2u, 2u, out2);
     return check_launch("inner product final");
@@ -428,7 +439,8 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z)
 template <class V>
 __global__ void __launch_bounds__(RT) k_sample(const V *__restrict__ s, uint32_t bb, uint64_t nblk,
                                                const double *__restrict__ cum, int64_t k, uint64_t seed,
-                                               const double *__restrict__ u_in, int64_t *__restrict__ out)
+                                               const double *__restrict__ u_in, int64_t *__restrict__ out,
+                                               StateLayout L)
 {
     const int64_t w = ((int64_t)blockIdx.x * RT + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
@@ -454,10 +466,11 @@ __global__ void __launch_bounds__(RT) k_sample(const V *__restrict__ s, uint32_t
     double base = b > 0 ? cum[b - 1] : 0.0;
     const uint32_t span = 1u << bb;
     const uint64_t first = b << bb;
+    const V *blk = s + elem(first, L);
     int64_t result = -1, last_nz = -1;
     for (uint32_t c0 = 0; c0 < span; c0 += 64) {
         const uint32_t j = c0 + lane;
-        const double p = j < span ? norm2(s[first + j]) : 0.0;
+        const double p = j < span ? norm2(blk[j]) : 0.0;
         double sc = p;
 #pragma unroll
         for (int off = 1; off < 64; off <<= 1) {
@@ -495,10 +508,10 @@ int launch_sample(int dtype, const void *state, int n, int64_t k, uint64_t seed,
     DMV_CHECK_ARG(blocks <= 0x7FFFFFFFull, "too many samples");
     if (dtype == DMV_C64)
         hipLaunchKernelGGL(k_sample<c64>, dim3((uint32_t)blocks), dim3(RT), 0, stream(), (const c64 *)state,
-                           (uint32_t)bb, nblk, block_cum, k, seed, u, out);
+                           (uint32_t)bb, nblk, block_cum, k, seed, u, out, state_layout(n, dtype == DMV_C64));
     else
         hipLaunchKernelGGL(k_sample<c128>, dim3((uint32_t)blocks), dim3(RT), 0, stream(), (const c128 *)state,
-                           (uint32_t)bb, nblk, block_cum, k, seed, u, out);
+                           (uint32_t)bb, nblk, block_cum, k, seed, u, out, state_layout(n, dtype == DMV_C64));
     return check_launch("sampler");
 }
 

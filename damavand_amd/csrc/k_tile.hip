@@ -18,7 +18,8 @@ namespace dmv {
 template <class R, int RLOG, int TLOG, bool NT>
 __global__ void __launch_bounds__(1 << TLOG)
 k_tile(typename Cplx<R>::T *__restrict__ state, const DevPass *__restrict__ passes,
-       const DevPhase *__restrict__ phases, const DevOp *__restrict__ ops, uint32_t pass_idx, uint32_t xcd_log2)
+       const DevPhase *__restrict__ phases, const DevOp *__restrict__ ops, uint32_t pass_idx, uint32_t xcd_log2,
+       StateLayout L)
 {
     using V = typename Cplx<R>::T;
     __shared__ V lds[1 << (RLOG + TLOG)];
@@ -28,7 +29,7 @@ k_tile(typename Cplx<R>::T *__restrict__ state, const DevPass *__restrict__ pass
     const uint32_t b = blockIdx.x;
     const uint64_t tile = xcd_log2 ? (uint64_t)(b & ((1u << xcd_log2) - 1)) * (gridDim.x >> xcd_log2) + (b >> xcd_log2)
                                    : (uint64_t)b;
-    tile_body<R, RLOG, TLOG, NT>(state, passes + pass_idx, phases, ops, tile, lds);
+    tile_body<R, RLOG, TLOG, NT>(state, passes + pass_idx, phases, ops, tile, lds, L);
 }
 
 // Persistent form: gridDim.x workgroups stride over the tiles with the next
@@ -36,11 +37,12 @@ k_tile(typename Cplx<R>::T *__restrict__ state, const DevPass *__restrict__ pass
 template <class R, int RLOG, int TLOG, bool NT>
 __global__ void __launch_bounds__(1 << TLOG)
 k_tile_pipe(typename Cplx<R>::T *__restrict__ state, const DevPass *__restrict__ passes,
-            const DevPhase *__restrict__ phases, const DevOp *__restrict__ ops, uint32_t pass_idx, uint64_t n_tiles)
+            const DevPhase *__restrict__ phases, const DevOp *__restrict__ ops, uint32_t pass_idx, uint64_t n_tiles,
+            StateLayout L)
 {
     using V = typename Cplx<R>::T;
     __shared__ V lds[1 << (RLOG + TLOG)];
-    tile_stream<R, RLOG, TLOG, NT>(state, passes + pass_idx, phases, ops, blockIdx.x, gridDim.x, n_tiles, lds);
+    tile_stream<R, RLOG, TLOG, NT>(state, passes + pass_idx, phases, ops, blockIdx.x, gridDim.x, n_tiles, lds, L);
 }
 
 struct PartialSink {
@@ -53,13 +55,13 @@ template <class R, int RLOG, int TLOG>
 __global__ void __launch_bounds__(1 << TLOG)
 k_tile_adjoint(typename Cplx<R>::T *__restrict__ psi, typename Cplx<R>::T *__restrict__ lam,
                const DevPass *__restrict__ passes, const DevPhase *__restrict__ phases,
-               const DevOp *__restrict__ ops, uint32_t pass_idx, double *__restrict__ partials)
+               const DevOp *__restrict__ ops, uint32_t pass_idx, double *__restrict__ partials, StateLayout L)
 {
     using V = typename Cplx<R>::T;
     __shared__ V lds[1 << (RLOG + TLOG)];
     __shared__ double red[(1 << TLOG) / 64];
     PartialSink sink{partials, gridDim.x, blockIdx.x};
-    tile_body_adjoint<R, RLOG, TLOG>(psi, lam, passes + pass_idx, phases, ops, blockIdx.x, lds, red, sink);
+    tile_body_adjoint<R, RLOG, TLOG>(psi, lam, passes + pass_idx, phases, ops, blockIdx.x, lds, red, sink, L);
 }
 
 // ------------------------------------------------------------- launchers --
@@ -69,6 +71,7 @@ static int run_tile(bool nt, void *state, int n, const DevPass *passes, const De
 {
     using V = typename Cplx<R>::T;
     const uint64_t tiles = uint64_t(1) << (n - (RLOG + TLOG));
+    const StateLayout L = state_layout(n, sizeof(R) == 4);
     const int64_t per_cu = tune_value("tile_wgs_per_cu", 2);
     const int64_t forced = tune_value("tile_grid", 0);  // explicit persistent grid (tests)
     const uint64_t grid64 = forced > 0 ? (uint64_t)forced : (uint64_t)device_cus() * (uint64_t)(per_cu > 0 ? per_cu : 1);
@@ -76,20 +79,20 @@ static int run_tile(bool nt, void *state, int n, const DevPass *passes, const De
         const uint32_t grid = (uint32_t)grid64;
         if (nt)
             hipLaunchKernelGGL((k_tile_pipe<R, RLOG, TLOG, true>), dim3(grid), dim3(1 << TLOG), 0, stream(),
-                               (V *)state, passes, phases, ops, pass_idx, tiles);
+                               (V *)state, passes, phases, ops, pass_idx, tiles, L);
         else
             hipLaunchKernelGGL((k_tile_pipe<R, RLOG, TLOG, false>), dim3(grid), dim3(1 << TLOG), 0, stream(),
-                               (V *)state, passes, phases, ops, pass_idx, tiles);
+                               (V *)state, passes, phases, ops, pass_idx, tiles, L);
         return check_launch("pipelined tile pass");
     }
     const int64_t xl = tune_value("tile_xcd_log2", 3);  // 8 XCDs per MI355X
     const uint32_t xcd_log2 = (xl > 0 && xl <= 4 && tiles >= (uint64_t(1) << xl)) ? (uint32_t)xl : 0u;
     if (nt)
         hipLaunchKernelGGL((k_tile<R, RLOG, TLOG, true>), dim3((uint32_t)tiles), dim3(1 << TLOG), 0, stream(),
-                           (V *)state, passes, phases, ops, pass_idx, xcd_log2);
+                           (V *)state, passes, phases, ops, pass_idx, xcd_log2, L);
     else
         hipLaunchKernelGGL((k_tile<R, RLOG, TLOG, false>), dim3((uint32_t)tiles), dim3(1 << TLOG), 0, stream(),
-                           (V *)state, passes, phases, ops, pass_idx, xcd_log2);
+                           (V *)state, passes, phases, ops, pass_idx, xcd_log2, L);
     return check_launch("tile pass");
 }
 
@@ -124,7 +127,8 @@ static int run_tile_adj(void *psi, void *lam, int n, const DevPass *passes, cons
     using V = typename Cplx<R>::T;
     const uint64_t tiles = uint64_t(1) << (n - (RLOG + TLOG));
     hipLaunchKernelGGL((k_tile_adjoint<R, RLOG, TLOG>), dim3((uint32_t)tiles), dim3(1 << TLOG), 0, stream(),
-                       (V *)psi, (V *)lam, passes, phases, ops, pass_idx, partials);
+                       (V *)psi, (V *)lam, passes, phases, ops, pass_idx, partials,
+                       state_layout(n, sizeof(R) == 4));
     return check_launch("adjoint tile pass");
 }
 
@@ -258,7 +262,7 @@ k_batch(const BatchCircuit *__restrict__ circ, const uint32_t *__restrict__ orde
         const uint64_t n_tiles = n_amp >> K;
         for (uint32_t p = C.fwd_begin; p < C.fwd_end; p++) {
             for (uint64_t t = 0; t < n_tiles; t++)
-                tile_body<R, RLOG, TLOG, false>(psi, passes + p, phases, ops, t, lds);
+                tile_body<R, RLOG, TLOG, false>(psi, passes + p, phases, ops, t, lds, kPlainLayout);
             __syncthreads();
         }
         double e = 0.0;
@@ -277,7 +281,8 @@ k_batch(const BatchCircuit *__restrict__ circ, const uint32_t *__restrict__ orde
         __syncthreads();
         for (uint32_t p = C.adj_begin; p < C.adj_end; p++) {
             for (uint64_t t = 0; t < n_tiles; t++)
-                tile_body_adjoint<R, RLOG, TLOG>(psi, lam, passes + p, phases, ops, t, lds, red, GradSink{g, t == 0});
+                tile_body_adjoint<R, RLOG, TLOG>(psi, lam, passes + p, phases, ops, t, lds, red, GradSink{g, t == 0},
+                                                 kPlainLayout);
             __syncthreads();
         }
     }

@@ -58,9 +58,11 @@ int launch_gate(int dtype, void *state, int n, const DevOp &op, int pairs_per_th
 int launch_init_basis(int dtype, void *state, int n, uint64_t basis);
 int launch_copy(void *dst, const void *src, uint64_t bytes);
 // Interleaved amplitudes [begin, begin+count) -> split re/im doubles (device).
-int launch_split(int dtype, const void *state, uint64_t begin, uint64_t count, double *re, double *im);
-int launch_join(int dtype, void *state, uint64_t begin, uint64_t count, const double *re, const double *im);
-int launch_probs(int dtype, const void *state, uint64_t begin, uint64_t count, double *out);
+int launch_split(int dtype, const void *state, StateLayout L, uint64_t begin, uint64_t count, double *re,
+                 double *im);
+int launch_join(int dtype, void *state, StateLayout L, uint64_t begin, uint64_t count, const double *re,
+                const double *im);
+int launch_probs(int dtype, const void *state, StateLayout L, uint64_t begin, uint64_t count, double *out);
 // Partial sums of |a|^2 over blocks of 2^block_bits amplitudes.
 int launch_block_norms(int dtype, const void *state, int n, int block_bits, double *partials);
 // Deterministic single-workgroup sum of `count` doubles into *out (device).

@@ -401,11 +401,18 @@ static int make_pass(uint64_t Q, const std::vector<HostOp> &taken, const PlanPar
     for (Ph &ph : phs)
         ph.rs = pad_regs(ph.rs, pp);
 
-    // Layouts: load/store layouts keep the low qubits in lanes.
+    // Layouts: load/store layouts keep the low qubits in lanes.  By default
+    // the first/last phase's layout is reused when it keeps the low qubits in
+    // lanes.  With memory-ordered layouts (tunable) they are always the
+    // canonical layout: lanes and waves take the tile's lowest qubits in
+    // ascending order and registers its highest, and a phase whose register
+    // set differs pays an LDS transpose instead (measured slower overall on
+    // the gapped state layout).
     const uint32_t default_rs = pad_regs(0u, pp);
+    const bool mem_order = tune_value("memory_layouts", 0) != 0;
     const bool first_coalesced = (phs.front().rs & lowmask) == 0;
     const bool last_coalesced = (phs.back().rs & lowmask) == 0;
-    P.lay_ld = make_layout(first_coalesced ? phs.front().rs : default_rs, pp, true);
+    P.lay_ld = make_layout(first_coalesced && !mem_order ? phs.front().rs : default_rs, pp, true);
     P.phase_begin = (uint32_t)plan->phases.size();
     const int pass_index = (int)plan->passes.size();
     const int NR = 1 << pp.rlog;
@@ -511,7 +518,8 @@ static int make_pass(uint64_t Q, const std::vector<HostOp> &taken, const PlanPar
     }
     P.phase_end = (uint32_t)plan->phases.size();
     const DevLayout last = prev;
-    const bool store_direct = last_coalesced && layouts_equal(last, make_layout(phs.back().rs, pp, true), pp);
+    const bool store_direct = mem_order ? layouts_equal(last, make_layout(default_rs, pp, true), pp)
+                                        : last_coalesced && layouts_equal(last, make_layout(phs.back().rs, pp, true), pp);
     P.lay_st = store_direct ? last : make_layout(default_rs, pp, true);
     P.transpose_st = layouts_equal(P.lay_st, last, pp) ? 0u : 1u;
     if (P.transpose_st) {
@@ -521,8 +529,9 @@ static int make_pass(uint64_t Q, const std::vector<HostOp> &taken, const PlanPar
     }
     // global offsets of the load/store layouts (the store through the final map)
     for (int r = 0; r < NR; r++) {
-        P.greg_ld[r] = deposit(P.lay_ld.lreg[r], P.qpos, pp.k);
-        P.greg_st[r] = deposit(P.lay_st.lreg[tab[r]], P.qpos, pp.k);
+        // element offsets: the kernels add them to the element of tile base | thread offset
+        P.greg_ld[r] = elem(deposit(P.lay_ld.lreg[r], P.qpos, pp.k), pp.layout);
+        P.greg_st[r] = elem(deposit(P.lay_st.lreg[tab[r]], P.qpos, pp.k), pp.layout);
     }
     for (int j = 0; j < pp.tlog; j++) {
         P.gthr_ld[j] = P.qpos[P.lay_ld.thr[j]];
@@ -655,7 +664,7 @@ int validate_plan(const Plan &plan, const PlanParams &pp)
                 return DMV_ERR_STATE;
             }
         for (int r = 0; r < (1 << pp.rlog); r++)
-            if (P.greg_ld[r] >= n_amp || P.greg_st[r] >= n_amp) {
+            if (P.greg_ld[r] >= state_elems(pp.n, pp.layout) || P.greg_st[r] >= state_elems(pp.n, pp.layout)) {
                 set_error("internal: register offset out of range");
                 return DMV_ERR_STATE;
             }

@@ -267,6 +267,7 @@ PlanParams tile_params(int n, int dtype, bool adjoint)
 {
     PlanParams pp;
     pp.n = n;
+    pp.layout = state_layout(n, dtype == DMV_C64);
     pp.low_bits = coalesce_bits(dtype);
     // wider contiguous runs per lane group (tunable; fewer free tile qubits)
     const int64_t runs = tune_value("coalesce_qubits", 0);
@@ -379,9 +380,10 @@ int state_create(int n, int dtype, dmv_state **out)
     s->n = n;
     s->dtype = dtype;
     s->bytes = (uint64_t(1) << n) * (uint64_t)amp_bytes(dtype);
+    s->alloc_bytes = state_elems(n, state_layout(n, dtype == DMV_C64)) * (uint64_t)amp_bytes(dtype);
     s->ws_doubles = ws_doubles_for(n);
     const uint64_t nblk = s->ws_doubles;
-    int rc = alloc(&s->amp, s->bytes);
+    int rc = alloc(&s->amp, s->alloc_bytes);
     if (rc == DMV_OK)
         rc = alloc((void **)&s->ws, s->ws_doubles * sizeof(double));
     if (rc == DMV_OK)
@@ -443,7 +445,7 @@ int state_get(dmv_state *s, double *re, double *im)
     double *dre, *dim;
     DMV_TRY(staging(&dre, &dim, chunk));
     for (uint64_t b = 0; b < n_amp; b += chunk) {
-        DMV_TRY(launch_split(s->dtype, s->amp, b, chunk, dre, dim));
+        DMV_TRY(launch_split(s->dtype, s->amp, state_layout(s->n, s->dtype == DMV_C64), b, chunk, dre, dim));
         if (re)
             DMV_TRY(copy_d2h(re + b, dre, chunk * sizeof(double)));
         if (im)
@@ -462,7 +464,7 @@ int state_set(dmv_state *s, const double *re, const double *im)
     for (uint64_t b = 0; b < n_amp; b += chunk) {
         DMV_TRY(copy_h2d(dre, re + b, chunk * sizeof(double)));
         DMV_TRY(copy_h2d(dim, im + b, chunk * sizeof(double)));
-        DMV_TRY(launch_join(s->dtype, s->amp, b, chunk, dre, dim));
+        DMV_TRY(launch_join(s->dtype, s->amp, state_layout(s->n, s->dtype == DMV_C64), b, chunk, dre, dim));
     }
     return synchronize();
 }
@@ -476,7 +478,7 @@ int state_probs(dmv_state *s, double *host)
     DMV_TRY(staging(&dre, &dim, chunk));
     for (uint64_t b = 0; b < n_amp; b += chunk) {
         prof_begin(DMV_KC_REDUCE);
-        DMV_TRY(launch_probs(s->dtype, s->amp, b, chunk, dre));
+        DMV_TRY(launch_probs(s->dtype, s->amp, state_layout(s->n, s->dtype == DMV_C64), b, chunk, dre));
         prof_end(DMV_KC_REDUCE, (double)chunk * (amp_bytes(s->dtype) + 8));
         DMV_TRY(copy_d2h(host + b, dre, chunk * sizeof(double)));
     }

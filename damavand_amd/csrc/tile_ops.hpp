@@ -174,6 +174,7 @@ __device__ __forceinline__ void op_gen_plain_m(V (&v)[NR], const Mat<R> &A)
         if (p & HB)
             continue;
         apply2(v[p], v[p ^ M], A);
+        __builtin_amdgcn_sched_barrier(0);
     }
 }
 
@@ -221,20 +222,27 @@ template <int K> __device__ __forceinline__ uint64_t tile_base(const DevPass *P,
     return g;
 }
 
+// g (tile base and thread offset) and the register offsets have disjoint
+// bits, so the element of amplitude g | reg is elem(g) + elem(reg); greg holds
+// the register offsets already mapped to elements by the planner.
 template <bool NT, class V, int NR>
-__device__ __forceinline__ void load_tile(V (&v)[NR], const V *__restrict__ state, uint64_t g, const uint64_t *greg)
+__device__ __forceinline__ void load_tile(V (&v)[NR], const V *__restrict__ state, uint64_t g, const uint64_t *greg,
+                                          StateLayout L)
 {
+    const V *base = state + elem(g, L);
 #pragma unroll
     for (int r = 0; r < NR; r++)
-        v[r] = ld<NT>(state + (g | greg[r]));
+        v[r] = ld<NT>(base + greg[r]);
 }
 
 template <bool NT, class V, int NR>
-__device__ __forceinline__ void store_tile(const V (&v)[NR], V *__restrict__ state, uint64_t g, const uint64_t *greg)
+__device__ __forceinline__ void store_tile(const V (&v)[NR], V *__restrict__ state, uint64_t g, const uint64_t *greg,
+                                           StateLayout L)
 {
+    V *base = state + elem(g, L);
 #pragma unroll
     for (int r = 0; r < NR; r++)
-        st<NT>(state + (g | greg[r]), v[r]);
+        st<NT>(base + greg[r], v[r]);
 }
 
 // Runs the phases of a pass on a loaded tile and leaves it in the store layout.
@@ -262,16 +270,16 @@ __device__ __forceinline__ void process_tile(typename Cplx<R>::T (&v)[1 << RLOG]
 template <class R, int RLOG, int TLOG, bool NT>
 __device__ __forceinline__ void tile_body(typename Cplx<R>::T *__restrict__ state, const DevPass *P,
                                           const DevPhase *__restrict__ phases, const DevOp *__restrict__ ops,
-                                          uint64_t tile, typename Cplx<R>::T *lds)
+                                          uint64_t tile, typename Cplx<R>::T *lds, StateLayout L)
 {
     using V = typename Cplx<R>::T;
     constexpr int NR = 1 << RLOG, K = RLOG + TLOG;
     const uint32_t tid = threadIdx.x;
     const uint64_t g = tile_base<K>(P, tile);
     V v[NR];
-    load_tile<NT>(v, state, g | thread_global_offset<TLOG>(P->gthr_ld, tid), P->greg_ld);
+    load_tile<NT>(v, state, g | thread_global_offset<TLOG>(P->gthr_ld, tid), P->greg_ld, L);
     process_tile<R, RLOG, TLOG>(v, P, phases, ops, lds);
-    store_tile<NT>(v, state, g | thread_global_offset<TLOG>(P->gthr_st, tid), P->greg_st);
+    store_tile<NT>(v, state, g | thread_global_offset<TLOG>(P->gthr_st, tid), P->greg_st, L);
 }
 
 // A workgroup's share of a pass, software pipelined: while tile t is
@@ -285,7 +293,7 @@ template <class R, int RLOG, int TLOG, bool NT>
 __device__ __forceinline__ void tile_stream(typename Cplx<R>::T *__restrict__ state, const DevPass *P,
                                             const DevPhase *__restrict__ phases, const DevOp *__restrict__ ops,
                                             uint64_t first, uint64_t stride, uint64_t n_tiles,
-                                            typename Cplx<R>::T *lds)
+                                            typename Cplx<R>::T *lds, StateLayout L)
 {
     using V = typename Cplx<R>::T;
     constexpr int NR = 1 << RLOG, K = RLOG + TLOG;
@@ -296,13 +304,13 @@ __device__ __forceinline__ void tile_stream(typename Cplx<R>::T *__restrict__ st
     uint64_t t = first;
     if (t >= n_tiles)
         return;
-    load_tile<NT>(a, state, tile_base<K>(P, t) | tl, P->greg_ld);
+    load_tile<NT>(a, state, tile_base<K>(P, t) | tl, P->greg_ld, L);
     for (;;) {
         const uint64_t t2 = t + stride;
         const bool more = t2 < n_tiles;
-        load_tile<NT>(b, state, tile_base<K>(P, more ? t2 : t) | tl, P->greg_ld);
+        load_tile<NT>(b, state, tile_base<K>(P, more ? t2 : t) | tl, P->greg_ld, L);
         process_tile<R, RLOG, TLOG>(a, P, phases, ops, lds);
-        store_tile<NT>(a, state, tile_base<K>(P, t) | ts, P->greg_st);
+        store_tile<NT>(a, state, tile_base<K>(P, t) | ts, P->greg_st, L);
         if (!more)
             break;
 #pragma unroll
@@ -373,7 +381,7 @@ __device__ __forceinline__ void tile_body_adjoint(typename Cplx<R>::T *__restric
                                                   typename Cplx<R>::T *__restrict__ lam, const DevPass *P,
                                                   const DevPhase *__restrict__ phases,
                                                   const DevOp *__restrict__ ops, uint64_t tile,
-                                                  typename Cplx<R>::T *lds, double *red, Sink sink)
+                                                  typename Cplx<R>::T *lds, double *red, Sink sink, StateLayout L)
 {
     using V = typename Cplx<R>::T;
     constexpr int NR = 1 << RLOG, K = RLOG + TLOG, NT = 1 << TLOG;
@@ -381,8 +389,8 @@ __device__ __forceinline__ void tile_body_adjoint(typename Cplx<R>::T *__restric
     const uint64_t g = tile_base<K>(P, tile);
     const uint64_t gl = g | thread_global_offset<TLOG>(P->gthr_ld, tid);
     V p[NR], l[NR];
-    load_tile<false>(p, psi, gl, P->greg_ld);
-    load_tile<false>(l, lam, gl, P->greg_ld);
+    load_tile<false>(p, psi, gl, P->greg_ld, L);
+    load_tile<false>(l, lam, gl, P->greg_ld, L);
     const DevLayout *cur = &P->lay_ld;
     for (uint32_t ph = P->phase_begin; ph < P->phase_end; ph++) {
         const DevPhase *D = phases + ph;
@@ -401,7 +409,10 @@ __device__ __forceinline__ void tile_body_adjoint(typename Cplx<R>::T *__restric
             }
             if (!thread_controls_hold(op, tid))
                 continue;
-            if (op->kind == OP_DIAG) {
+            if (op->plain) {
+                op_gen_plain<V, R>(p, op);
+                op_gen_plain<V, R>(l, op);
+            } else if (op->kind == OP_DIAG) {
                 op_diag<V, R>(p, op, tid);
                 op_diag<V, R>(l, op, tid);
             } else {
@@ -415,8 +426,8 @@ __device__ __forceinline__ void tile_body_adjoint(typename Cplx<R>::T *__restric
         transpose<V, NR, TLOG>(l, lds, cur, P->st_src_lswz, &P->lay_st, tid);
     }
     const uint64_t gs = g | thread_global_offset<TLOG>(P->gthr_st, tid);
-    store_tile<false>(p, psi, gs, P->greg_st);
-    store_tile<false>(l, lam, gs, P->greg_st);
+    store_tile<false>(p, psi, gs, P->greg_st, L);
+    store_tile<false>(l, lam, gs, P->greg_st, L);
 }
 
 // ------------------------------------------------------ whole-state in LDS --

@@ -154,8 +154,17 @@ int dmv_state_get(dmv_state *s, double *re, double *im);
 int dmv_state_set(dmv_state *s, const double *re, const double *im);
 /* dst <- src (same size and dtype). */
 int dmv_state_copy(dmv_state *dst, const dmv_state *src);
-/* Raw device pointer of the amplitude buffer and its byte size. */
+/* Raw device pointer of the amplitude buffer and its byte size (allocation,
+ * gaps included).  Amplitude i is element i + (i >> shift) * gap of the
+ * buffer, with shift and gap from dmv_state_layout: the buffer skips a small
+ * gap after every 2 MiB of amplitudes so that strided passes spread over the
+ * HBM channels.  States below 2^18 (complex128) / 2^19 (complex64) amplitudes
+ * have no gaps.  dmv_state_get/dmv_state_set always use plain arrays. */
 int dmv_state_device_ptr(dmv_state *s, void **ptr, uint64_t *bytes);
+/* The element map of dmv_state_device_ptr: *shift = log2 of the amplitudes
+ * between gaps, *gap = amplitudes skipped per gap (0: plain layout).
+ * No counterpart in the reference (its states are plain split re/im arrays). */
+int dmv_state_layout(const dmv_state *s, int *shift, int *gap);
 
 /* ---- single operations (one kernel launch each) ---- */
 /* Row-major 2x2 (re[4], im[4]) on target, controlled on every qubit of

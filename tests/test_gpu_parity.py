@@ -411,6 +411,44 @@ def test_hipgraph_replays(native):
         N.tune_set("graph", 0)
 
 
+@pytest.mark.parametrize("dtype,n", [("complex128", 19), ("complex64", 20)])
+def test_state_layout_with_gaps(native, dtype, n):
+    """States past 2 MiB of amplitudes skip a gap per span: the reported
+    layout and size match, and loading, fused and per-gate passes over high
+    qubits, reductions and readback all agree with the oracle."""
+    say(f"case state layout {dtype} n={n}")
+    c64 = dtype == "complex64"
+    tol = 1e-5 if c64 else TOL
+    sv = StateVector(n, dtype)
+    shift, gap = ctypes.c_int(0), ctypes.c_int(0)
+    N.call("dmv_state_layout", sv.handle, ctypes.byref(shift), ctypes.byref(gap))
+    assert (shift.value, gap.value) == ((18, 128) if c64 else (17, 64))
+    ptr, nbytes = ctypes.c_void_p(), ctypes.c_uint64(0)
+    N.call("dmv_state_device_ptr", sv.handle, ctypes.byref(ptr), ctypes.byref(nbytes))
+    elems = (1 << n) + (((1 << n) - 1) >> shift.value) * gap.value
+    assert nbytes.value == elems * (8 if c64 else 16)
+    rng = np.random.default_rng(n)
+    amp = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    amp /= np.linalg.norm(amp)
+    sv.load(amp)
+    np.testing.assert_allclose(sv.amplitudes(), amp, atol=1e-7 if c64 else 0)
+    gates = [(wl.GATE_H, -1, n - 1, ()), (wl.GATE_CNOT, n - 1, n - 2, ()), (wl.GATE_RX, -1, n - 3, (0.7,)),
+             (wl.GATE_CNOT, 0, n - 1, ()), (wl.GATE_H, -1, 1, ()), (wl.GATE_RZ, -1, n - 1, (1.1,))]
+    ref = oracle.OracleState(n)
+    ref.amp[:] = amp
+    for kind, ctl, t, p in gates:
+        ref.apply_gate(kind, t, ctl, p)
+    for per_gate in (False, True):
+        c = build_circuit(n, gates, dtype)
+        c.state.load(amp)
+        c.forward(per_gate=per_gate)
+        err = np.max(np.abs(c.retrieve_amplitudes_on_host() - ref.amp))
+        say(f"  per_gate={per_gate} max-abs error {err:.3e}")
+        assert err <= tol
+        np.testing.assert_allclose(c.state.probabilities(), np.abs(ref.amp) ** 2, atol=tol)
+        np.testing.assert_allclose(c.state.expect_z(), ref.expect_z(), atol=tol * 10)
+
+
 def test_errors(native):
     c = Circuit(3, apply_method="gpu")
     with pytest.raises(N.DamavandError):

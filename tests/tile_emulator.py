@@ -161,10 +161,17 @@ class TileEmulator:
             v[:, tm, p] = a00 * x + a01 * y
             v[:, tm, q] = a10 * x + a11 * y
 
+    def element(self, i):
+        # element of amplitude i in the device layout: a gap of `gap` after every 2^shift amplitudes
+        shift, gap = self.plan["layout"]
+        return i + (i >> shift) * gap if gap else i
+
     def run_pass(self, P, psi, lam=None, sink=None):
+        # psi / lam are device-layout buffers; addresses are formed as the kernels form them:
+        # elem(tile base | thread offset) + greg[r], greg already mapped to elements by the planner
         g = self.tile_bases(P)
         gl = self.thread_global(P["gthr_ld"])
-        idx_ld = g[:, None, None] | gl[None, :, None] | np.array(P["greg_ld"], dtype=np.int64)[None, None, :]
+        idx_ld = self.element(g[:, None, None] | gl[None, :, None]) + np.array(P["greg_ld"], dtype=np.int64)[None, None, :]
         assert idx_ld.max() < psi.size
         v = psi[idx_ld].copy()
         lv = lam[idx_ld].copy() if lam is not None else None
@@ -187,26 +194,35 @@ class TileEmulator:
             if lv is not None:
                 lv = self.transpose(lv, cur, P["st_src_lswz"], P["lay_st"])
         gs = self.thread_global(P["gthr_st"])
-        idx_st = g[:, None, None] | gs[None, :, None] | np.array(P["greg_st"], dtype=np.int64)[None, None, :]
+        idx_st = self.element(g[:, None, None] | gs[None, :, None]) + np.array(P["greg_st"], dtype=np.int64)[None, None, :]
         flat = np.sort(idx_st.reshape(-1))
-        assert np.array_equal(flat, np.arange(psi.size)), "stores do not cover the state exactly once"
+        assert np.array_equal(flat, self.element(np.arange(1 << self.n))), "stores do not cover the state exactly once"
         psi[idx_st] = v
         if lam is not None:
             lam[idx_st] = lv
 
+    def to_device(self, amp):
+        e = self.element(np.arange(amp.size))
+        buf = np.full(int(e[-1]) + 1, np.nan + 0j)
+        buf[e] = amp
+        return buf
+
+    def from_device(self, buf):
+        return buf[self.element(np.arange(1 << self.n))]
+
     def forward(self, psi):
-        psi = psi.copy()
+        buf = self.to_device(psi)
         for P in self.plan["passes"]:
-            self.run_pass(P, psi)
-        return psi
+            self.run_pass(P, buf)
+        return self.from_device(buf)
 
     def reverse(self, psi, lam):
-        psi, lam = psi.copy(), lam.copy()
+        bp, bl = self.to_device(psi), self.to_device(lam)
         grads = np.zeros(self.plan["n_slots"])
 
         def sink(slot, acc):
             grads[slot] += acc.sum()
 
         for P in self.plan["passes"]:
-            self.run_pass(P, psi, lam, sink)
-        return psi, lam, grads
+            self.run_pass(P, bp, bl, sink)
+        return self.from_device(bp), self.from_device(bl), grads
