@@ -64,7 +64,7 @@ int dmv_state_bytes(int num_qubits, int dtype, uint64_t *bytes)
 {
     DMV_CHECK_ARG(bytes != nullptr, "null pointer");
     DMV_CHECK_ARG(num_qubits >= 1 && num_qubits <= kMaxQubits, "num_qubits out of range");
-    const uint64_t amp = state_elems(num_qubits, state_layout(num_qubits, dtype == DMV_C64)) * (uint64_t)amp_bytes(dtype);
+    const uint64_t amp = (uint64_t(1) << num_qubits) * (uint64_t)amp_bytes(dtype);
     const int B = num_qubits < 12 ? num_qubits : 12;
     const uint64_t nblk = uint64_t(1) << (num_qubits - B);
     *bytes = amp + (nblk * 13 + 64) * 8 * 2 + 64 * 8;
@@ -165,14 +165,14 @@ int dmv_state_device_ptr(dmv_state *s, void **ptr, uint64_t *bytes)
     return DMV_OK;
 }
 
-int dmv_state_layout(const dmv_state *s, int *shift, int *gap)
+int dmv_state_layout(const dmv_state *s, int *shift, int *block_bits)
 {
     CHECK_STATE(s);
     const StateLayout L = state_layout(s->n, s->dtype == DMV_C64);
     if (shift)
         *shift = (int)L.shift;
-    if (gap)
-        *gap = (int)L.gap;
+    if (block_bits)
+        *block_bits = (int)L.blk;
     return DMV_OK;
 }
 
@@ -689,7 +689,7 @@ int dmv_circuit_plan_json(dmv_circuit *c, int dtype, int adjoint, int tile_qubit
     std::string o = "{";
     o += "\"n\":" + std::to_string(pp.n) + ",\"k\":" + std::to_string(pp.k) + ",\"rlog\":" + std::to_string(pp.rlog) +
          ",\"tlog\":" + std::to_string(pp.tlog) + ",\"n_slots\":" + std::to_string(plan.n_slots) +
-         ",\"layout\":[" + std::to_string(pp.layout.shift) + "," + std::to_string(pp.layout.gap) + "],\"passes\":[";
+         ",\"layout\":[" + std::to_string(pp.layout.shift) + "," + std::to_string(pp.layout.blk) + "],\"passes\":[";
     for (size_t i = 0; i < plan.passes.size(); i++) {
         const DevPass &P = plan.passes[i];
         if (i)

@@ -40,7 +40,7 @@ k_gate(typename Cplx<R>::T *__restrict__ state, uint64_t n_items, uint32_t t, Po
         idx[u] = expand(p, P) | ormask;
         e[u] = elem(idx[u], L);
     }
-    // idx has bit t clear, so the partner's element is e + elem(2^t)
+    // idx has bit t clear, so the partner's element is e ^ elem(2^t)
     const uint64_t tb = elem(uint64_t(1) << t, L);
     if constexpr (MODE == GM_GEN || MODE == GM_SWAP) {
         V a[PPT], b[PPT];
@@ -48,7 +48,7 @@ k_gate(typename Cplx<R>::T *__restrict__ state, uint64_t n_items, uint32_t t, Po
         for (int u = 0; u < PPT; u++)
             if (ok[u]) {
                 a[u] = ld<NT>(state + e[u]);
-                b[u] = ld<NT>(state + (e[u] + tb));
+                b[u] = ld<NT>(state + (e[u] ^ tb));
             }
 #pragma unroll
         for (int u = 0; u < PPT; u++)
@@ -56,10 +56,10 @@ k_gate(typename Cplx<R>::T *__restrict__ state, uint64_t n_items, uint32_t t, Po
                 if constexpr (MODE == GM_GEN) {
                     apply2(a[u], b[u], M);
                     st<NT>(state + e[u], a[u]);
-                    st<NT>(state + (e[u] + tb), b[u]);
+                    st<NT>(state + (e[u] ^ tb), b[u]);
                 } else {
                     st<NT>(state + e[u], b[u]);
-                    st<NT>(state + (e[u] + tb), a[u]);
+                    st<NT>(state + (e[u] ^ tb), a[u]);
                 }
             }
     } else {

@@ -137,11 +137,11 @@ __global__ void __launch_bounds__(RT) k_block_norms(const V *__restrict__ s, uin
                                                    StateLayout L)
 {
     __shared__ double red[RT / 64];
-    const V *blk = s + elem((uint64_t)blockIdx.x << bb, L);  // a block never spans a gap
+    const uint64_t eb = elem((uint64_t)blockIdx.x << bb, L);  // a block lies in one span: element eb ^ i
     const uint32_t span = 1u << bb;
     double acc = 0.0;
     for (uint32_t i = threadIdx.x; i < span; i += RT)
-        acc += norm2(blk[i]);
+        acc += norm2(s[eb ^ i]);
     acc = block_sum<RT>(acc, red);
     if (threadIdx.x == 0)
         partials[blockIdx.x] = acc;
@@ -150,7 +150,7 @@ __global__ void __launch_bounds__(RT) k_block_norms(const V *__restrict__ s, uin
 int launch_block_norms(int dtype, const void *state, int n, int bb, double *partials)
 {
     const StateLayout L = state_layout(n, dtype == DMV_C64);
-    DMV_CHECK_ARG(bb >= 0 && bb <= n && n - bb <= 31 && (L.gap == 0 || bb <= (int)L.shift), "block bits out of range");
+    DMV_CHECK_ARG(bb >= 0 && bb <= n && n - bb <= 31 && (L.shift == 0 || bb <= (int)L.shift), "block bits out of range");
     const uint32_t blocks = 1u << (n - bb);
     if (dtype == DMV_C64)
         hipLaunchKernelGGL(k_block_norms<c64>, dim3(blocks), dim3(RT), 0, stream(), (const c64 *)state, (uint32_t)bb,
@@ -194,14 +194,14 @@ __global__ void __launch_bounds__(RT) k_expect_z_part(const V *__restrict__ s, u
                                                      StateLayout L)
 {
     __shared__ double red[RT / 64];
-    const V *blk = s + elem((uint64_t)blockIdx.x << B, L);
+    const uint64_t eb = elem((uint64_t)blockIdx.x << B, L);
     const uint32_t span = 1u << B;
     double tot = 0.0, m[kBlockBits];
 #pragma unroll
     for (int q = 0; q < kBlockBits; q++)
         m[q] = 0.0;
     for (uint32_t i = threadIdx.x; i < span; i += RT) {
-        const double p = norm2(blk[i]);
+        const double p = norm2(s[eb ^ i]);
         tot += p;
 #pragma unroll
         for (int q = 0; q < kBlockBits; q++)
@@ -302,13 +302,13 @@ __global__ void __launch_bounds__(RT) k_obs(const V *__restrict__ s, V *__restri
             h += ((x >> q) & 1u) ? -sa[q] : sa[q];
         for (int t = 0; t < T.n_pairs; t++)
             h += (__popcll(x & smask[t]) & 1) ? -sb[t] : sb[t];
-        const V v = s[e0 + i];
+        const V v = s[e0 ^ i];
         acc += h * norm2(v);
         if constexpr (WRITE) {
             V w;
             w.x = v.x * h;
             w.y = v.y * h;
-            lam[e0 + i] = w;
+            lam[e0 ^ i] = w;
         }
     }
     acc = block_sum<RT>(acc, red);
@@ -356,7 +356,7 @@ __global__ void __launch_bounds__(RT) k_inner(const V *__restrict__ a, const V *
     const uint32_t span = 1u << B;
     double re = 0.0, im = 0.0;
     for (uint32_t i = threadIdx.x; i < span; i += RT) {
-        const V x = a[base + i], y = b[base + i];
+        const V x = a[base ^ i], y = b[base ^ i];
         re += (double)x.x * y.x + (double)x.y * y.y;
         im += (double)x.x * y.y - (double)x.y * y.x;
     }
@@ -466,11 +466,11 @@ __global__ void __launch_bounds__(RT) k_sample(const V *__restrict__ s, uint32_t
     double base = b > 0 ? cum[b - 1] : 0.0;
     const uint32_t span = 1u << bb;
     const uint64_t first = b << bb;
-    const V *blk = s + elem(first, L);
+    const uint64_t eb = elem(first, L);
     int64_t result = -1, last_nz = -1;
     for (uint32_t c0 = 0; c0 < span; c0 += 64) {
         const uint32_t j = c0 + lane;
-        const double p = j < span ? norm2(blk[j]) : 0.0;
+        const double p = j < span ? norm2(s[eb ^ j]) : 0.0;
         double sc = p;
 #pragma unroll
         for (int off = 1; off < 64; off <<= 1) {

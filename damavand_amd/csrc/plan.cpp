@@ -664,7 +664,7 @@ int validate_plan(const Plan &plan, const PlanParams &pp)
                 return DMV_ERR_STATE;
             }
         for (int r = 0; r < (1 << pp.rlog); r++)
-            if (P.greg_ld[r] >= state_elems(pp.n, pp.layout) || P.greg_st[r] >= state_elems(pp.n, pp.layout)) {
+            if (P.greg_ld[r] >= n_amp || P.greg_st[r] >= n_amp) {
                 set_error("internal: register offset out of range");
                 return DMV_ERR_STATE;
             }

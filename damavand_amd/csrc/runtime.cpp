@@ -380,7 +380,7 @@ int state_create(int n, int dtype, dmv_state **out)
     s->n = n;
     s->dtype = dtype;
     s->bytes = (uint64_t(1) << n) * (uint64_t)amp_bytes(dtype);
-    s->alloc_bytes = state_elems(n, state_layout(n, dtype == DMV_C64)) * (uint64_t)amp_bytes(dtype);
+    s->alloc_bytes = s->bytes;  // the swizzled layout is a permutation within the plain size
     s->ws_doubles = ws_doubles_for(n);
     const uint64_t nblk = s->ws_doubles;
     int rc = alloc(&s->amp, s->alloc_bytes);

@@ -13,7 +13,7 @@ struct dmv_state {
     int dtype = DMV_C128;
     void *amp = nullptr;          // 2^n interleaved amplitudes in the layout of state_layout(n, dtype)
     uint64_t bytes = 0;           // 2^n amplitudes' worth (algorithmic size)
-    uint64_t alloc_bytes = 0;     // allocation, gaps included
+    uint64_t alloc_bytes = 0;     // allocation size
     double *ws = nullptr;         // reduction partials
     uint64_t ws_doubles = 0;
     double *cum = nullptr;        // sampler block prefix

@@ -223,26 +223,26 @@ template <int K> __device__ __forceinline__ uint64_t tile_base(const DevPass *P,
 }
 
 // g (tile base and thread offset) and the register offsets have disjoint
-// bits, so the element of amplitude g | reg is elem(g) + elem(reg); greg holds
+// bits, so the element of amplitude g | reg is elem(g) ^ elem(reg); greg holds
 // the register offsets already mapped to elements by the planner.
 template <bool NT, class V, int NR>
 __device__ __forceinline__ void load_tile(V (&v)[NR], const V *__restrict__ state, uint64_t g, const uint64_t *greg,
                                           StateLayout L)
 {
-    const V *base = state + elem(g, L);
+    const uint64_t eg = elem(g, L);
 #pragma unroll
     for (int r = 0; r < NR; r++)
-        v[r] = ld<NT>(base + greg[r]);
+        v[r] = ld<NT>(state + (eg ^ greg[r]));
 }
 
 template <bool NT, class V, int NR>
 __device__ __forceinline__ void store_tile(const V (&v)[NR], V *__restrict__ state, uint64_t g, const uint64_t *greg,
                                            StateLayout L)
 {
-    V *base = state + elem(g, L);
+    const uint64_t eg = elem(g, L);
 #pragma unroll
     for (int r = 0; r < NR; r++)
-        st<NT>(base + greg[r], v[r]);
+        st<NT>(state + (eg ^ greg[r]), v[r]);
 }
 
 // Runs the phases of a pass on a loaded tile and leaves it in the store layout.

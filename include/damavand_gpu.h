@@ -154,17 +154,18 @@ int dmv_state_get(dmv_state *s, double *re, double *im);
 int dmv_state_set(dmv_state *s, const double *re, const double *im);
 /* dst <- src (same size and dtype). */
 int dmv_state_copy(dmv_state *dst, const dmv_state *src);
-/* Raw device pointer of the amplitude buffer and its byte size (allocation,
- * gaps included).  Amplitude i is element i + (i >> shift) * gap of the
- * buffer, with shift and gap from dmv_state_layout: the buffer skips a small
- * gap after every 2 MiB of amplitudes so that strided passes spread over the
- * HBM channels.  States below 2^18 (complex128) / 2^19 (complex64) amplitudes
- * have no gaps.  dmv_state_get/dmv_state_set always use plain arrays. */
+/* Raw device pointer of the amplitude buffer (2^n amplitudes) and its byte
+ * size.  The order is swizzled (dmv_state_layout) so that strided passes
+ * spread over the HBM channels; dmv_state_get/dmv_state_set always use plain
+ * arrays. */
 int dmv_state_device_ptr(dmv_state *s, void **ptr, uint64_t *bytes);
-/* The element map of dmv_state_device_ptr: *shift = log2 of the amplitudes
- * between gaps, *gap = amplitudes skipped per gap (0: plain layout).
+/* The element order of dmv_state_device_ptr.  *shift = 0: plain order.
+ * Otherwise amplitude i is element i ^ (P(i >> shift) << block_bits), where
+ * P maps bit j of its argument to bit (7 j mod 11) (XOR-linear); shift = 17,
+ * block_bits = 6 for complex128 (2 MiB spans of 1 KiB blocks), 18 and 7 for
+ * complex64, on states of more than one span.
  * No counterpart in the reference (its states are plain split re/im arrays). */
-int dmv_state_layout(const dmv_state *s, int *shift, int *gap);
+int dmv_state_layout(const dmv_state *s, int *shift, int *block_bits);
 
 /* ---- single operations (one kernel launch each) ---- */
 /* Row-major 2x2 (re[4], im[4]) on target, controlled on every qubit of

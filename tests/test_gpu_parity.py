@@ -412,21 +412,20 @@ def test_hipgraph_replays(native):
 
 
 @pytest.mark.parametrize("dtype,n", [("complex128", 19), ("complex64", 20)])
-def test_state_layout_with_gaps(native, dtype, n):
-    """States past 2 MiB of amplitudes skip a gap per span: the reported
+def test_state_layout_swizzled(native, dtype, n):
+    """States past 2 MiB of amplitudes have the swizzled order: the reported
     layout and size match, and loading, fused and per-gate passes over high
     qubits, reductions and readback all agree with the oracle."""
     say(f"case state layout {dtype} n={n}")
     c64 = dtype == "complex64"
     tol = 1e-5 if c64 else TOL
     sv = StateVector(n, dtype)
-    shift, gap = ctypes.c_int(0), ctypes.c_int(0)
-    N.call("dmv_state_layout", sv.handle, ctypes.byref(shift), ctypes.byref(gap))
-    assert (shift.value, gap.value) == ((18, 128) if c64 else (17, 64))
+    shift, blk = ctypes.c_int(0), ctypes.c_int(0)
+    N.call("dmv_state_layout", sv.handle, ctypes.byref(shift), ctypes.byref(blk))
+    assert (shift.value, blk.value) == ((18, 7) if c64 else (17, 6))
     ptr, nbytes = ctypes.c_void_p(), ctypes.c_uint64(0)
     N.call("dmv_state_device_ptr", sv.handle, ctypes.byref(ptr), ctypes.byref(nbytes))
-    elems = (1 << n) + (((1 << n) - 1) >> shift.value) * gap.value
-    assert nbytes.value == elems * (8 if c64 else 16)
+    assert nbytes.value == (1 << n) * (8 if c64 else 16)
     rng = np.random.default_rng(n)
     amp = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
     amp /= np.linalg.norm(amp)

@@ -136,15 +136,16 @@ def test_adjoint_plan_emulation(n, layers, k, tlog):
 
 
 def test_gapped_layout_emulation():
-    """A state past 2 MiB of amplitudes: the planner's element offsets and the
-    kernels' element map of tile base | thread offset address exactly the
-    state's elements (stores cover each once, results match the oracle)."""
+    """A state past 2 MiB of amplitudes (swizzled order): the planner's element
+    offsets and the kernels' element map of tile base | thread offset address
+    exactly the state's elements (stores cover each once, results match the
+    oracle)."""
     n = 19
     rng = np.random.default_rng(31)
     gates = random_gates(rng, n, 120) + [(wl.GATE_H, -1, q, ()) for q in range(12, n)]
     nc = NativeCircuit(n, gates)
     plan = plan_json(nc.h, tile_qubits=12, tlog=8, max_gen=8)
-    assert plan["layout"] == [17, 64]
+    assert plan["layout"] == [17, 6]
     assert max(max(P["greg_ld"]) for P in plan["passes"]) >= (1 << 17)
     got = TileEmulator(plan).forward(zero_state(n))
     np.testing.assert_allclose(got, oracle.run_gates(n, gates).amp, atol=1e-12)

@@ -162,16 +162,23 @@ class TileEmulator:
             v[:, tm, q] = a10 * x + a11 * y
 
     def element(self, i):
-        # element of amplitude i in the device layout: a gap of `gap` after every 2^shift amplitudes
-        shift, gap = self.plan["layout"]
-        return i + (i >> shift) * gap if gap else i
+        # element of amplitude i in the device layout: within each span of 2^shift amplitudes the
+        # block index (blocks of 2^blk) is XORed with P(span), P: bit j -> bit (7 j mod 11)
+        shift, blk = self.plan["layout"]
+        if not shift:
+            return i
+        span = np.asarray(i) >> shift
+        p = np.zeros_like(span)
+        for j in range(16):
+            p ^= ((span >> j) & 1) << ((7 * j) % 11)
+        return i ^ (p << blk)
 
     def run_pass(self, P, psi, lam=None, sink=None):
         # psi / lam are device-layout buffers; addresses are formed as the kernels form them:
-        # elem(tile base | thread offset) + greg[r], greg already mapped to elements by the planner
+        # elem(tile base | thread offset) ^ greg[r], greg already mapped to elements by the planner
         g = self.tile_bases(P)
         gl = self.thread_global(P["gthr_ld"])
-        idx_ld = self.element(g[:, None, None] | gl[None, :, None]) + np.array(P["greg_ld"], dtype=np.int64)[None, None, :]
+        idx_ld = self.element(g[:, None, None] | gl[None, :, None]) ^ np.array(P["greg_ld"], dtype=np.int64)[None, None, :]
         assert idx_ld.max() < psi.size
         v = psi[idx_ld].copy()
         lv = lam[idx_ld].copy() if lam is not None else None
@@ -194,17 +201,16 @@ class TileEmulator:
             if lv is not None:
                 lv = self.transpose(lv, cur, P["st_src_lswz"], P["lay_st"])
         gs = self.thread_global(P["gthr_st"])
-        idx_st = self.element(g[:, None, None] | gs[None, :, None]) + np.array(P["greg_st"], dtype=np.int64)[None, None, :]
+        idx_st = self.element(g[:, None, None] | gs[None, :, None]) ^ np.array(P["greg_st"], dtype=np.int64)[None, None, :]
         flat = np.sort(idx_st.reshape(-1))
-        assert np.array_equal(flat, self.element(np.arange(1 << self.n))), "stores do not cover the state exactly once"
+        assert np.array_equal(flat, np.arange(1 << self.n)), "stores do not cover the state exactly once"
         psi[idx_st] = v
         if lam is not None:
             lam[idx_st] = lv
 
     def to_device(self, amp):
-        e = self.element(np.arange(amp.size))
-        buf = np.full(int(e[-1]) + 1, np.nan + 0j)
-        buf[e] = amp
+        buf = np.full(amp.size, np.nan + 0j)
+        buf[self.element(np.arange(amp.size))] = amp
         return buf
 
     def from_device(self, buf):
