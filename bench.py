@@ -241,6 +241,23 @@ def bench_cpu_baseline(n, layers, sample_gates):
                       f"circuit, complex128, {t:.1f} s ({t / len(gates):.2f} s/gate; state init {t_init:.1f} s)"}
 
 
+def measured_traffic(qubits, layers):
+    """HBM bytes per fused-pass launch from the committed rocprofv3 counter
+    summary (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the
+    factor the same counter reads on the copy kernel), or None when the
+    summary does not cover this workload."""
+    path = os.path.join(ROOT, "profiles", "k_tile_30q_layered_c128_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("qubits") != qubits or t.get("layers") != layers:
+        return None
+    return (t["fetch_kib_per_dispatch"] * 1024.0 * t["fetch_factor"] +
+            t["write_kib_per_dispatch"] * 1024.0 * t["write_factor"])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -302,7 +319,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "achieved": round(fw["achieved"], 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-            "frac": round(fw["achieved"] / HBM_PEAK_GBPS, 4), "traffic": None,
+            "frac": round(fw["achieved"] / HBM_PEAK_GBPS, 4), "traffic": measured_traffic(args.qubits, args.layers),
             "kernel": "k_tile (fused tile pass)", "launches": fw["launches"],
             "avg_launch_ms": round(fw["avg_launch_ms"], 4), "bytes_per_launch": fw["per_launch_bytes"],
         },
