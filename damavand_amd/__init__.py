@@ -8,7 +8,8 @@ runs in libdamavand-gpu.so (hand-written HIP kernels for gfx950); build it with
 """
 from ._native import DamavandError, NoDeviceError, OutOfDeviceMemory  # noqa: F401
 from .circuit import APPLY_METHODS, Circuit, StateVector, initialize_mpi  # noqa: F401
-from .batch import batch_forward_adjoint  # noqa: F401
+from .batch import PackedBatch, batch_forward_adjoint, batch_forward_adjoint_packed, pack_circuits  # noqa: F401
 
-__all__ = ["Circuit", "StateVector", "initialize_mpi", "batch_forward_adjoint", "DamavandError",
+__all__ = ["Circuit", "StateVector", "initialize_mpi", "batch_forward_adjoint", "batch_forward_adjoint_packed",
+           "pack_circuits", "PackedBatch", "DamavandError",
            "NoDeviceError", "OutOfDeviceMemory", "APPLY_METHODS"]

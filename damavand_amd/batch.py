@@ -11,21 +11,35 @@ from .circuit import _dtype_code
 PARAMETRIC = (N.GATE_RX, N.GATE_RY, N.GATE_RZ)
 
 
-def batch_forward_adjoint(circuits: Sequence, dtype="complex128"):
-    """Runs every circuit from |0>, evaluates its chain observable and its
-    adjoint gradient, all in one persistent kernel launch.
+class PackedBatch:
+    """Circuits of a batch in the flat arrays of dmv_batch_forward_adjoint
+    (include/damavand_gpu.h): per-circuit qubit counts, gate / coefficient /
+    parameter offsets, and the gates and coefficients of all circuits."""
 
-    ``circuits`` is a sequence of ``(n, gates, a, b)``: ``gates`` a list of
-    ``(kind, control, target, params)``, ``a`` the n Z coefficients and ``b``
-    the n-1 nearest-neighbour ZZ coefficients of
-    H = sum_q a_q Z_q + sum_{q<n-1} b_q Z_q Z_{q+1}.
+    def __init__(self, n, gate_offset, kinds, ctrls, tgts, params, coef_offset, coef, param_offset, n_params):
+        self.n = np.ascontiguousarray(n, dtype=np.int32)
+        self.gate_offset = np.ascontiguousarray(gate_offset, dtype=np.int32)
+        self.kinds = np.ascontiguousarray(kinds, dtype=np.int32)
+        self.ctrls = np.ascontiguousarray(ctrls, dtype=np.int32)
+        self.tgts = np.ascontiguousarray(tgts, dtype=np.int32)
+        self.params = np.ascontiguousarray(params, dtype=np.float64).reshape(-1)
+        self.coef_offset = np.ascontiguousarray(coef_offset, dtype=np.int32)
+        self.coef = np.ascontiguousarray(coef, dtype=np.float64)
+        self.param_offset = np.ascontiguousarray(param_offset, dtype=np.int32)
+        self.n_params = np.ascontiguousarray(n_params, dtype=np.int64)
 
-    Returns ``(energies, grads)``: an array of <H> per circuit and a list of
-    gradient arrays (one entry per RX/RY/RZ gate, in insertion order).
-    """
+    @property
+    def count(self) -> int:
+        return int(self.n.size)
+
+    @property
+    def total_params(self) -> int:
+        return int(self.n_params.sum())
+
+
+def pack_circuits(circuits: Sequence) -> PackedBatch:
+    """``circuits``: a sequence of ``(n, gates, a, b)`` (see batch_forward_adjoint)."""
     count = len(circuits)
-    if count == 0:
-        return np.zeros(0), []
     n = np.zeros(count, dtype=np.int32)
     gate_offset = np.zeros(count + 1, dtype=np.int32)
     coef_offset = np.zeros(count, dtype=np.int32)
@@ -59,15 +73,37 @@ def batch_forward_adjoint(circuits: Sequence, dtype="complex128"):
         p_total += np_i
         n_params.append(np_i)
     gate_offset[count] = g_total
-    kinds = np.ascontiguousarray(kinds, dtype=np.int32)
-    ctrls = np.ascontiguousarray(ctrls, dtype=np.int32)
-    tgts = np.ascontiguousarray(tgts, dtype=np.int32)
-    params = np.ascontiguousarray(np.asarray(params, dtype=np.float64).reshape(-1))
-    coef = np.ascontiguousarray(np.concatenate(coefs))
-    energy = np.empty(count)
-    grad = np.empty(max(1, p_total))
-    N.call("dmv_batch_forward_adjoint", count, _dtype_code(dtype), N.iptr(n), N.iptr(gate_offset), N.iptr(kinds),
-           N.iptr(ctrls), N.iptr(tgts), N.dptr(params), N.iptr(coef_offset), N.dptr(coef), N.iptr(param_offset),
-           N.dptr(energy), N.dptr(grad))
-    grads = [grad[param_offset[i]: param_offset[i] + n_params[i]].copy() for i in range(count)]
+    return PackedBatch(n, gate_offset, kinds, ctrls, tgts, np.asarray(params, dtype=np.float64).reshape(-1),
+                       coef_offset, np.concatenate(coefs) if coefs else np.zeros(0), param_offset, n_params)
+
+
+def batch_forward_adjoint_packed(batch: PackedBatch, dtype="complex128"):
+    """One persistent launch over a packed batch: returns (energies, flat
+    gradient array in the order of ``batch.param_offset``)."""
+    energy = np.empty(batch.count)
+    grad = np.empty(max(1, batch.total_params))
+    N.call("dmv_batch_forward_adjoint", batch.count, _dtype_code(dtype), N.iptr(batch.n), N.iptr(batch.gate_offset),
+           N.iptr(batch.kinds), N.iptr(batch.ctrls), N.iptr(batch.tgts), N.dptr(batch.params),
+           N.iptr(batch.coef_offset), N.dptr(batch.coef), N.iptr(batch.param_offset), N.dptr(energy), N.dptr(grad))
+    return energy, grad[: batch.total_params]
+
+
+def batch_forward_adjoint(circuits: Sequence, dtype="complex128"):
+    """Runs every circuit from |0>, evaluates its chain observable and its
+    adjoint gradient, all in one persistent kernel launch.
+
+    ``circuits`` is a sequence of ``(n, gates, a, b)``: ``gates`` a list of
+    ``(kind, control, target, params)``, ``a`` the n Z coefficients and ``b``
+    the n-1 nearest-neighbour ZZ coefficients of
+    H = sum_q a_q Z_q + sum_{q<n-1} b_q Z_q Z_{q+1}.
+
+    Returns ``(energies, grads)``: an array of <H> per circuit and a list of
+    gradient arrays (one entry per RX/RY/RZ gate, in insertion order).
+    """
+    if len(circuits) == 0:
+        return np.zeros(0), []
+    batch = pack_circuits(circuits)
+    energy, grad = batch_forward_adjoint_packed(batch, dtype)
+    grads = [grad[batch.param_offset[i]: batch.param_offset[i] + batch.n_params[i]].copy()
+             for i in range(batch.count)]
     return energy, grads

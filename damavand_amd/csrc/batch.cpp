@@ -1,12 +1,14 @@
 // Batched forward + adjoint of many small circuits in one persistent launch,
 // and the autotuner of the fused tile shape.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "device.hpp"
@@ -33,17 +35,29 @@ int batch_forward_adjoint(int count, int dtype, const int *n, const int *gate_of
     const int K = rlog + tlog;
     const int small_max = K - 1;
 
-    std::vector<BatchCircuit> circ(count);
-    std::vector<DevPass> passes;
-    std::vector<DevPhase> phases;
-    std::vector<DevOp> ops, sops;
-    std::vector<double> cost(count);
-    int nmax = 1;
-    int64_t total_params = 0;
-    for (int i = 0; i < count; i++) {
+    // Per-circuit plans are independent: build them on host threads, then
+    // concatenate in circuit order (the result does not depend on threading).
+    struct Built {
+        int rc = DMV_OK;
+        std::string err;
+        int np = 0;
+        double cost = 0.0;
+        bool small = false;
+        std::vector<DevOp> fo, ao;
+        Plan pf, pa;
+    };
+    std::vector<Built> built(count);
+    PlanParams pp0;
+    pp0.k = K;
+    pp0.rlog = rlog;
+    pp0.tlog = tlog;
+    pp0.low_bits = coalesce_bits(dtype);
+    pp0.swz_bits = dtype == DMV_C64 ? 4 : 3;
+    pp0.max_gen_per_pass = (int)tune_value("max_gen_per_pass", 8);
+    auto build_one = [&](int i, Built &B) -> int {
         const int ni = n[i];
         DMV_CHECK_ARG(ni >= 2 && ni <= 24, "batch circuit %d: qubits must be in [2, 24], got %d", i, ni);
-        nmax = std::max(nmax, ni);
+        DMV_CHECK_ARG(coef_offset[i] >= 0, "bad coefficient offset");
         std::vector<Gate> gates;
         for (int g = gate_offset[i]; g < gate_offset[i + 1]; g++) {
             Gate G;
@@ -63,40 +77,75 @@ int batch_forward_adjoint(int count, int dtype, const int *n, const int *gate_of
                 G.control = -1;
             gates.push_back(G);
         }
-        int np = 0;
         for (const Gate &G : gates)
             if (gate_is_parametric(G.kind))
-                np++;
-        total_params += np;
+                B.np++;
         std::vector<HostOp> f = merge_ops(forward_ops(gates));
         std::vector<HostOp> a = merge_ops(adjoint_ops(gates));
+        B.cost = (double)(f.size() + a.size()) * std::ldexp(1.0, ni);
+        B.small = ni <= small_max;
+        if (B.small) {
+            B.fo = global_dev_ops(f);
+            B.ao = global_dev_ops(a);
+            return DMV_OK;
+        }
+        PlanParams pp = pp0;
+        pp.n = ni;
+        DMV_TRY(build_plan(f, pp, &B.pf));
+        return build_plan(a, pp, &B.pa);
+    };
+    {
+        const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
+        const int nthreads = std::max(1, std::min({hw, 16, (count + 31) / 32}));
+        std::atomic<int> next{0};
+        auto worker = [&]() {
+            for (int i = next.fetch_add(1); i < count; i = next.fetch_add(1)) {
+                built[i].rc = build_one(i, built[i]);
+                if (built[i].rc != DMV_OK)
+                    built[i].err = last_error();
+            }
+        };
+        std::vector<std::thread> pool;
+        for (int t = 1; t < nthreads; t++)
+            pool.emplace_back(worker);
+        worker();
+        for (std::thread &t : pool)
+            t.join();
+    }
+
+    std::vector<BatchCircuit> circ(count);
+    std::vector<DevPass> passes;
+    std::vector<DevPhase> phases;
+    std::vector<DevOp> ops, sops;
+    std::vector<double> cost(count);
+    int nmax = 1;
+    int64_t total_params = 0;
+    for (int i = 0; i < count; i++) {
+        Built &B = built[i];
+        if (B.rc != DMV_OK) {
+            set_error("%s", B.err.c_str());
+            return B.rc;
+        }
+        const int ni = n[i];
+        nmax = std::max(nmax, ni);
+        total_params += B.np;
         BatchCircuit &C = circ[i];
         C.n = (uint32_t)ni;
         C.coef = (uint32_t)coef_offset[i];
         C.grad = (uint32_t)param_offset[i];
-        cost[i] = (double)(f.size() + a.size()) * std::ldexp(1.0, ni);
-        if (ni <= small_max) {
+        cost[i] = B.cost;
+        if (B.small) {
             C.small = 1;
-            std::vector<DevOp> fo = global_dev_ops(f), ao = global_dev_ops(a);
             C.fwd_begin = (uint32_t)sops.size();
-            sops.insert(sops.end(), fo.begin(), fo.end());
+            sops.insert(sops.end(), B.fo.begin(), B.fo.end());
             C.fwd_end = (uint32_t)sops.size();
             C.adj_begin = (uint32_t)sops.size();
-            sops.insert(sops.end(), ao.begin(), ao.end());
+            sops.insert(sops.end(), B.ao.begin(), B.ao.end());
             C.adj_end = (uint32_t)sops.size();
         } else {
             C.small = 0;
-            PlanParams pp;
-            pp.n = ni;
-            pp.k = K;
-            pp.rlog = rlog;
-            pp.tlog = tlog;
-            pp.low_bits = coalesce_bits(dtype);
-            pp.swz_bits = dtype == DMV_C64 ? 4 : 3;
-            pp.max_gen_per_pass = (int)tune_value("max_gen_per_pass", 8);
             for (int which = 0; which < 2; which++) {
-                Plan plan;
-                DMV_TRY(build_plan(which == 0 ? f : a, pp, &plan));
+                const Plan &plan = which == 0 ? B.pf : B.pa;
                 const uint32_t pbase = (uint32_t)passes.size();
                 const uint32_t phbase = (uint32_t)phases.size();
                 const uint32_t obase = (uint32_t)ops.size();
@@ -120,20 +169,15 @@ int batch_forward_adjoint(int count, int dtype, const int *n, const int *gate_of
                 }
             }
         }
-        DMV_CHECK_ARG(coef_offset[i] >= 0, "bad coefficient offset");
+        B = Built();  // release the per-circuit copies early
     }
     std::vector<uint32_t> order(count);
     for (int i = 0; i < count; i++)
         order[i] = (uint32_t)i;
     std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return cost[x] > cost[y]; });
 
-    hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, 0) != hipSuccess) {
-        set_error("hipGetDeviceProperties failed");
-        return DMV_ERR_HIP;
-    }
     const int per_cu = (int)tune_value("batch_wgs_per_cu", 2);
-    uint32_t grid = (uint32_t)std::min<int64_t>(count, (int64_t)prop.multiProcessorCount * std::max(1, per_cu));
+    uint32_t grid = (uint32_t)std::min<int64_t>(count, (int64_t)device_cus() * std::max(1, per_cu));
     const uint64_t slot_amps = uint64_t(1) << std::max(nmax, K);
     const uint64_t scratch_bytes = (uint64_t)grid * 2 * slot_amps * amp_bytes(dtype);
 

@@ -544,7 +544,8 @@ int dmv_circuit_adjoint(dmv_circuit *c, dmv_state *psi, dmv_state *lam, const do
     // reverse sweep
     DMV_TRY(prepare_plan(c->adj, c->version, c->n, psi->dtype, true, adjoint_ops(c->gates)));
     DevicePlan &dp = c->adj;
-    const uint64_t n_tiles = dp.small ? 1 : (uint64_t(1) << (c->n - dp.pp.k));
+    // partials per slot: one per wave of every tile (tile path), one (whole-state path)
+    const uint64_t n_tiles = dp.small ? 1 : (uint64_t(1) << (c->n - dp.pp.k)) * ((uint64_t(1) << dp.pp.tlog) / 64);
     const int n_slots = dp.plan.n_slots;
     const uint64_t need = (uint64_t)(n_slots > 0 ? n_slots : 1) * n_tiles;
     if (need > c->cap_partials) {

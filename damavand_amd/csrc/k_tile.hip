@@ -45,10 +45,15 @@ k_tile_pipe(typename Cplx<R>::T *__restrict__ state, const DevPass *__restrict__
     tile_stream<R, RLOG, TLOG, NT>(state, passes + pass_idx, phases, ops, blockIdx.x, gridDim.x, n_tiles, lds, L);
 }
 
+// partials[slot][tile][wave]: one value per wave of every tile.
 struct PartialSink {
+    static constexpr bool kPerWave = true;
     double *partials;
-    uint64_t n_tiles, tile;
-    __device__ void operator()(int slot, double v) const { partials[(uint64_t)slot * n_tiles + tile] = v; }
+    uint64_t per_slot, base;  // per_slot = tiles x waves, base = tile x waves
+    __device__ void operator()(int slot, double v, uint32_t wave) const
+    {
+        partials[(uint64_t)slot * per_slot + base + wave] = v;
+    }
 };
 
 template <class R, int RLOG, int TLOG>
@@ -60,7 +65,8 @@ k_tile_adjoint(typename Cplx<R>::T *__restrict__ psi, typename Cplx<R>::T *__res
     using V = typename Cplx<R>::T;
     __shared__ V lds[1 << (RLOG + TLOG)];
     __shared__ double red[(1 << TLOG) / 64];
-    PartialSink sink{partials, gridDim.x, blockIdx.x};
+    constexpr uint32_t W = (1u << TLOG) / 64;
+    PartialSink sink{partials, (uint64_t)gridDim.x * W, (uint64_t)blockIdx.x * W};
     tile_body_adjoint<R, RLOG, TLOG>(psi, lam, passes + pass_idx, phases, ops, blockIdx.x, lds, red, sink, L);
 }
 
@@ -154,9 +160,10 @@ int launch_tile_adjoint(int dtype, int rlog, int tlog, void *psi, void *lam, int
 
 // ---------------------------------------------------------------- batched --
 struct GradSink {
+    static constexpr bool kPerWave = false;
     double *grad;
     bool first;  // first tile of the pass: store, later tiles: accumulate
-    __device__ void operator()(int slot, double v) const
+    __device__ void operator()(int slot, double v, uint32_t) const
     {
         if (first)
             grad[slot] = v;

@@ -375,7 +375,8 @@ __device__ __forceinline__ double ip_registers(const V (&p)[1 << RLOG], const V 
 }
 
 // One tile of a reverse-sweep pass over psi and lambda.  For every IP op the
-// workgroup sum is handed to sink(slot, value) in thread 0.
+// sum over each wave (Sink::kPerWave) or over the workgroup is handed to
+// sink(slot, value, wave) by the first lane of the wave / thread 0.
 template <class R, int RLOG, int TLOG, class Sink>
 __device__ __forceinline__ void tile_body_adjoint(typename Cplx<R>::T *__restrict__ psi,
                                                   typename Cplx<R>::T *__restrict__ lam, const DevPass *P,
@@ -402,9 +403,16 @@ __device__ __forceinline__ void tile_body_adjoint(typename Cplx<R>::T *__restric
         for (uint32_t o = D->op_begin; o < D->op_end; o++) {
             const DevOp *op = ops + o;
             if (op_is_ip(op->kind)) {
-                const double s = block_sum<NT>(ip_registers<V, RLOG>(p, l, op, tid), red);
-                if (tid == 0)
-                    sink(op->slot, s);
+                if constexpr (Sink::kPerWave) {
+                    // one partial per wave: shuffles only, no workgroup barrier
+                    const double w = wave_sum(ip_registers<V, RLOG>(p, l, op, tid));
+                    if ((tid & 63) == 0)
+                        sink(op->slot, w, tid >> 6);
+                } else {
+                    const double s = block_sum<NT>(ip_registers<V, RLOG>(p, l, op, tid), red);
+                    if (tid == 0)
+                        sink(op->slot, s, 0);
+                }
                 continue;
             }
             if (!thread_controls_hold(op, tid))

@@ -174,3 +174,26 @@ def test_expectation_signs_follow_reference():
     samples = np.array([0, 1, 2, 3, 5])
     out = expectation_signs(samples, [0, 2, 0])
     np.testing.assert_array_equal(out, [[1, 1, 1], [-1, 1, -1], [1, 1, 1], [-1, 1, -1], [-1, -1, -1]])
+
+
+def test_pack_circuits_offsets():
+    """The flat arrays of the batch ABI: offsets, parameter rows, coefficients."""
+    from damavand_amd.batch import pack_circuits
+    from damavand_amd import workloads as wl
+    circuits = wl.batch_workload(5, 77)
+    b = pack_circuits(circuits)
+    assert b.count == 5
+    g = 0
+    c = 0
+    p = 0
+    for i, (ni, gates, a, bb) in enumerate(circuits):
+        assert b.n[i] == ni and b.gate_offset[i] == g and b.coef_offset[i] == c and b.param_offset[i] == p
+        for j, (kind, ctl, tgt, prm) in enumerate(gates):
+            assert (b.kinds[g + j], b.ctrls[g + j], b.tgts[g + j]) == (kind, ctl, tgt)
+            np.testing.assert_array_equal(b.params[3 * (g + j): 3 * (g + j) + len(prm)], prm)
+        np.testing.assert_array_equal(b.coef[c: c + ni], a)
+        np.testing.assert_array_equal(b.coef[c + ni: c + 2 * ni - 1], bb)
+        g += len(gates)
+        c += 2 * ni - 1
+        p += wl.count_parameters(gates)
+    assert b.gate_offset[5] == g and b.total_params == p
