@@ -71,8 +71,11 @@ def build(jobs: int = 0, verbose: bool = False) -> str:
     os.makedirs(OBJ, exist_ok=True)
     srcs = sources()
     jobs = jobs or min(8, os.cpu_count() or 1)
+    # longest compiles first (the kernel instantiation units), so they overlap
+    order = sorted(srcs, key=lambda s: -os.path.getsize(s) if s.endswith(".hip") else 0)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(lambda s: _compile(s, verbose), srcs))
+        done = dict(zip(order, ex.map(lambda s: _compile(s, verbose), order)))
+    objs = [done[s] for s in srcs]
     tmp = LIB + ".tmp"
     cmd = [hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs
     if verbose:

@@ -1,0 +1,14 @@
+// Fused tile passes, complex128, the default 16-register shapes.
+#include "k_tile_impl.hpp"
+
+namespace dmv {
+
+int launch_tile_c128_main(int rlog, int tlog, bool nt, void *state, int n, const DevPass *passes, const DevPhase *phases,
+                          const DevOp *ops, uint32_t pass_idx)
+{
+    if (tlog == 9 && rlog == 4) return run_tile<double, 4, 9>(nt, state, n, passes, phases, ops, pass_idx);
+    if (tlog == 8 && rlog == 4) return run_tile<double, 4, 8>(nt, state, n, passes, phases, ops, pass_idx);
+    return kNotHandled;
+}
+
+}  // namespace dmv
