@@ -237,10 +237,27 @@ def bench_cpu_baseline(n, layers, sample_gates):
         st.apply_gate(kind, tgt, ctl, p)
     t = time.perf_counter() - t0
     algo = sum(wl.algorithmic_bytes_per_gate(g, n) for g in gates)
-    del st
-    return {"value": algo / t / 1e9, "unit": "GB/s", "cores": oracle.num_threads(), "kind": "port",
+    cores = oracle.num_threads()
+    # one more gate on a single thread (the reference's per-gate pass without the thread pool)
+    oracle.set_num_threads(1)
+    g1 = wl.layered_circuit(n, layers, 1002)[sample_gates]
+    t0 = time.perf_counter()
+    st.apply_gate(g1[0], g1[2], g1[1], g1[3])
+    t1 = time.perf_counter() - t0
+    oracle.set_num_threads(cores)
+    # the reference sampler (utils.rs:258-277): one (N+1)-long cumulative rebuilt per draw
+    probs = st.probabilities()
+    t0 = time.perf_counter()
+    oracle.sample_uniforms(probs, np.array([0.25, 0.75]), rebuild=True)
+    t_sample = (time.perf_counter() - t0) / 2
+    del st, probs
+    return {"value": algo / t / 1e9, "unit": "GB/s", "cores": cores, "kind": "port",
             "sample": f"first {len(gates)} gates (H on qubits 0..{len(gates) - 1}) of the {n}-qubit layered "
-                      f"circuit, complex128, {t:.1f} s ({t / len(gates):.2f} s/gate; state init {t_init:.1f} s)"}
+                      f"circuit, complex128, {t:.1f} s ({t / len(gates):.2f} s/gate; state init {t_init:.1f} s)",
+            "value_1_thread": wl.algorithmic_bytes_per_gate(g1, n) / t1 / 1e9,
+            "sample_1_thread": f"gate {sample_gates} of the same circuit on one thread, {t1:.1f} s",
+            "sampler_s_per_draw": t_sample,
+            "sampler_note": f"reference sampler at {n} qubits, cumulative rebuilt per draw, mean of 2 draws"}
 
 
 def measured_traffic(qubits, layers):
