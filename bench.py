@@ -302,7 +302,7 @@ def main():
 
     from damavand_amd import _native as N
     log(f"[bench] {N.version()}, devices {N.device_count()}")
-    copy_gbps = bench_copy(4 << 30)
+    copy_gbps = bench_copy(16 << 30)  # a state-sized copy: 16 GiB read + 16 GiB written
     log(f"[bench] streaming copy ceiling {copy_gbps:.0f} GB/s")
     fw, circ, gates = bench_forward(args.qubits, args.layers, args.steps, args.warmup)
     per_gate = None if args.skip_per_gate else bench_per_gate(circ)
