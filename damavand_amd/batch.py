@@ -36,6 +36,28 @@ class PackedBatch:
     def total_params(self) -> int:
         return int(self.n_params.sum())
 
+    def parameters(self) -> np.ndarray:
+        """The RX/RY/RZ angles of all circuits, circuit after circuit in gate
+        order: the order of the gradients."""
+        return self.params.reshape(-1, 3)[self._param_rows(), 0].copy()
+
+    def with_parameters(self, theta) -> "PackedBatch":
+        """A copy whose RX/RY/RZ angles are ``theta`` (same order as
+        ``parameters()``); gates, coefficients and offsets are shared."""
+        theta = np.ascontiguousarray(theta, dtype=np.float64).reshape(-1)
+        rows = self._param_rows()
+        if theta.size != rows.size:
+            raise ValueError(f"need {rows.size} angles, got {theta.size}")
+        params = self.params.reshape(-1, 3).copy()
+        params[rows, 0] = theta
+        out = object.__new__(PackedBatch)
+        out.__dict__.update(self.__dict__)
+        out.params = params.reshape(-1)
+        return out
+
+    def _param_rows(self) -> np.ndarray:
+        return np.flatnonzero(np.isin(self.kinds, PARAMETRIC))
+
 
 def pack_circuits(circuits: Sequence) -> PackedBatch:
     """``circuits``: a sequence of ``(n, gates, a, b)`` (see batch_forward_adjoint)."""

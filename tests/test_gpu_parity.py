@@ -459,3 +459,61 @@ def test_errors(native):
         st.apply_matrix(np.eye(2), 5)
     with pytest.raises(N.DamavandError):
         st.apply_matrix(np.eye(2), 1, controls=[1])
+
+
+def test_torch_autograd_single_and_batched(native):
+    """The torch bridges: expectation() of one circuit and batch_expectation()
+    of a packed batch, energies and theta.grad against the oracle (energy and
+    parameter shift)."""
+    import torch
+    from damavand_amd.autograd import batch_expectation, expectation
+    from damavand_amd.batch import pack_circuits
+    say("case torch autograd")
+    n = 8
+    gates, a, b = wl.ansatz_circuit(n, 3, 77)
+    pairs = wl.chain_pairs(n)
+    c = build_circuit(n, gates)
+    theta = torch.tensor([g[3][0] for g in gates if g[0] in wl.PARAMETRIC], dtype=torch.float64, requires_grad=True)
+    e = expectation(c, theta, a, pairs, b)
+    e.backward()
+    st = oracle.run_gates(n, gates)
+    assert abs(e.item() - st.expect_zz(a, pairs, b)) <= 1e-11
+    pidx = [i for i, g in enumerate(gates) if g[0] in wl.PARAMETRIC]
+    ref = []
+    for gi in pidx:
+        vals = []
+        for s in (0.5 * math.pi, -0.5 * math.pi):
+            g2 = list(gates)
+            k, ctl, t, p = g2[gi]
+            g2[gi] = (k, ctl, t, (p[0] + s,))
+            vals.append(oracle.run_gates(n, g2).expect_zz(a, pairs, b))
+        ref.append(0.5 * (vals[0] - vals[1]))
+    np.testing.assert_allclose(theta.grad.numpy(), ref, atol=1e-10)
+
+    circuits = wl.batch_workload(6, 88, n_range=(9, 13), layer_range=(2, 3))  # LDS and tile paths
+    packed = pack_circuits(circuits)
+    rng = np.random.default_rng(5)
+    th = torch.tensor(rng.uniform(0, 2 * math.pi, packed.total_params), requires_grad=True)
+    w = torch.tensor(rng.normal(size=packed.count))
+    energies = batch_expectation(packed, th)
+    (energies * w).sum().backward()
+    # per circuit: energy and gradient at the substituted angles, on the oracle
+    off = 0
+    for i, (ni, g0, ai, bi) in enumerate(circuits):
+        prm = th.detach().numpy()[off: off + packed.n_params[i]]
+        it = iter(prm)
+        gi = [(k, ctl, t, (next(it),) if k in wl.PARAMETRIC else p) for (k, ctl, t, p) in g0]
+        chain = wl.chain_pairs(ni)
+        ref_e = oracle.run_gates(ni, gi).expect_zz(ai, chain, bi)
+        assert abs(energies[i].item() - ref_e) <= 1e-10
+        # two angles of each circuit by parameter shift
+        for j in (0, len(prm) - 1):
+            pos = [q for q, g in enumerate(gi) if g[0] in wl.PARAMETRIC][j]
+            vals = []
+            for s in (0.5 * math.pi, -0.5 * math.pi):
+                g2 = list(gi)
+                k, ctl, t, p = g2[pos]
+                g2[pos] = (k, ctl, t, (p[0] + s,))
+                vals.append(oracle.run_gates(ni, g2).expect_zz(ai, chain, bi))
+            assert abs(th.grad[off + j].item() - w[i].item() * 0.5 * (vals[0] - vals[1])) <= 1e-9
+        off += packed.n_params[i]

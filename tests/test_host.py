@@ -197,3 +197,21 @@ def test_pack_circuits_offsets():
         c += 2 * ni - 1
         p += wl.count_parameters(gates)
     assert b.gate_offset[5] == g and b.total_params == p
+
+
+def test_packed_batch_parameter_substitution():
+    """with_parameters replaces exactly the RX/RY/RZ angles, in gradient order."""
+    from damavand_amd.batch import pack_circuits
+    from damavand_amd import workloads as wl
+    circuits = wl.batch_workload(4, 78)
+    b = pack_circuits(circuits)
+    theta0 = b.parameters()
+    assert theta0.size == b.total_params
+    expect = [p[0] for _, gates, _, _ in circuits for (k, c, t, p) in gates if k in wl.PARAMETRIC]
+    np.testing.assert_array_equal(theta0, expect)
+    theta = np.arange(b.total_params, dtype=float) * 0.01
+    b2 = b.with_parameters(theta)
+    np.testing.assert_array_equal(b2.parameters(), theta)
+    rows = np.isin(b.kinds, list(wl.PARAMETRIC))
+    np.testing.assert_array_equal(b2.params.reshape(-1, 3)[~rows], b.params.reshape(-1, 3)[~rows])
+    np.testing.assert_array_equal(b.parameters(), theta0)  # the original is untouched
