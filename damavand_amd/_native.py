@@ -64,7 +64,8 @@ _SIGNATURES = {
     "dmv_state_set": (ctypes.c_int, [_vp, _c_dbl_p, _c_dbl_p]),
     "dmv_state_copy": (ctypes.c_int, [_vp, _vp]),
     "dmv_state_device_ptr": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), _c_u64_p]),
-    "dmv_state_layout": (ctypes.c_int, [_vp, _c_int_p, _c_int_p]),
+    "dmv_state_layout": (ctypes.c_int, [_vp, _c_int_p, _c_int_p, _c_int_p]),
+    "dmv_state_pieces": (ctypes.c_int, [_vp, _c_int_p]),
     "dmv_apply_matrix": (ctypes.c_int, [_vp, _c_dbl_p, _c_dbl_p, ctypes.c_uint64, ctypes.c_int]),
     "dmv_apply_gate": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_dbl_p]),
     "dmv_probabilities": (ctypes.c_int, [_vp, _c_dbl_p]),

@@ -148,6 +148,7 @@ int dmv_state_copy(dmv_state *dst, const dmv_state *src)
     CHECK_STATE(dst);
     CHECK_STATE(src);
     DMV_CHECK_ARG(dst->n == src->n && dst->dtype == src->dtype, "states differ in size or dtype");
+    DMV_CHECK_ARG(same_layout(dst->layout, src->layout), "states differ in element order (created under other settings)");
     DMV_TRY(ensure_device());
     prof_begin(DMV_KC_COPY);
     DMV_TRY(launch_copy(dst->amp, src->amp, dst->alloc_bytes));
@@ -165,14 +166,24 @@ int dmv_state_device_ptr(dmv_state *s, void **ptr, uint64_t *bytes)
     return DMV_OK;
 }
 
-int dmv_state_layout(const dmv_state *s, int *shift, int *block_bits)
+int dmv_state_pieces(const dmv_state *s, int *pieces)
 {
     CHECK_STATE(s);
-    const StateLayout L = state_layout(s->n, s->dtype == DMV_C64);
+    DMV_CHECK_ARG(pieces != nullptr, "null output pointer");
+    *pieces = s->spread.va ? kSpreadPieces : 1;
+    return DMV_OK;
+}
+
+int dmv_state_layout(const dmv_state *s, int *shift, int *block_bits, int *fold_bits)
+{
+    CHECK_STATE(s);
+    const StateLayout L = s->layout;
     if (shift)
         *shift = (int)L.shift;
     if (block_bits)
         *block_bits = (int)L.blk;
+    if (fold_bits)
+        *fold_bits = (int)L.fold;
     return DMV_OK;
 }
 
@@ -209,7 +220,7 @@ int dmv_apply_matrix(dmv_state *s, const double *m_re, const double *m_im, uint6
     const bool phase_only = op.kind == OP_DIAG && op.m[0] == 1.0 && op.m[1] == 0.0;
     const double touched = (double)(n_amp >> (popcount64(control_mask) + (phase_only ? 1 : 0)));
     prof_begin(DMV_KC_GATE);
-    DMV_TRY(launch_gate(s->dtype, s->amp, s->n, op, gate_pairs_per_thread(), gate_nontemporal(op.target, s->dtype)));
+    DMV_TRY(launch_gate(s->dtype, s->amp, s->n, s->layout, op, gate_pairs_per_thread(), gate_nontemporal(op.target, s->dtype)));
     prof_end(DMV_KC_GATE, 2.0 * touched * amp_bytes(s->dtype));
     return DMV_OK;
 }
@@ -243,7 +254,7 @@ int dmv_norm_sqr(dmv_state *s, double *out)
     DMV_TRY(ensure_device());
     const int B = s->n < 12 ? s->n : 12;
     prof_begin(DMV_KC_REDUCE);
-    DMV_TRY(launch_block_norms(s->dtype, s->amp, s->n, B, s->ws));
+    DMV_TRY(launch_block_norms(s->dtype, s->amp, s->n, s->layout, B, s->ws));
     DMV_TRY(launch_sum(s->ws, uint64_t(1) << (s->n - B), s->scalars));
     prof_end(DMV_KC_REDUCE, (double)s->bytes);
     return copy_d2h(out, s->scalars, sizeof(double));
@@ -255,7 +266,7 @@ int dmv_expect_z(dmv_state *s, double *out)
     DMV_CHECK_ARG(out != nullptr, "null output");
     DMV_TRY(ensure_device());
     prof_begin(DMV_KC_REDUCE);
-    DMV_TRY(launch_expect_z(s->dtype, s->amp, s->n, s->ws, s->scalars));
+    DMV_TRY(launch_expect_z(s->dtype, s->amp, s->n, s->layout, s->ws, s->scalars));
     prof_end(DMV_KC_REDUCE, (double)s->bytes);
     return copy_d2h(out, s->scalars, sizeof(double) * s->n);
 }
@@ -307,7 +318,7 @@ int dmv_expect_zz(dmv_state *s, const double *a, int n_pairs, const int *pi, con
     const int *dpi, *dpj;
     DMV_TRY(upload_terms(&terms, &cap, s->n, a, n_pairs, pi, pj, b, &da, &db, &dpi, &dpj));
     prof_begin(DMV_KC_REDUCE);
-    DMV_TRY(launch_expect_zz(s->dtype, s->amp, s->n, da, n_pairs, dpi, dpj, db, s->ws, s->scalars));
+    DMV_TRY(launch_expect_zz(s->dtype, s->amp, s->n, s->layout, da, n_pairs, dpi, dpj, db, s->ws, s->scalars));
     prof_end(DMV_KC_REDUCE, (double)s->bytes);
     return copy_d2h(out, s->scalars, sizeof(double));
 }
@@ -318,9 +329,10 @@ int dmv_inner_product(dmv_state *a, dmv_state *b, double *out_re_im)
     CHECK_STATE(b);
     DMV_CHECK_ARG(out_re_im != nullptr, "null output");
     DMV_CHECK_ARG(a->n == b->n && a->dtype == b->dtype, "states differ in size or dtype");
+    DMV_CHECK_ARG(same_layout(a->layout, b->layout), "states differ in element order (created under other settings)");
     DMV_TRY(ensure_device());
     prof_begin(DMV_KC_REDUCE);
-    DMV_TRY(launch_inner(a->dtype, a->amp, b->amp, a->n, a->ws, a->scalars));
+    DMV_TRY(launch_inner(a->dtype, a->amp, b->amp, a->n, a->layout, a->ws, a->scalars));
     prof_end(DMV_KC_REDUCE, 2.0 * (double)a->bytes);
     return copy_d2h(out_re_im, a->scalars, 2 * sizeof(double));
 }
@@ -351,7 +363,7 @@ static int sample_impl(dmv_state *s, int64_t k, uint64_t seed, const double *u_h
     if (rc == DMV_OK) {
         // the reference panics when it cannot sample (circuit.rs:478-482): refuse a zero state
         const int bb = s->n < 12 ? s->n : 12;
-        rc = launch_block_norms(s->dtype, s->amp, s->n, bb, s->ws);
+        rc = launch_block_norms(s->dtype, s->amp, s->n, s->layout, bb, s->ws);
         if (rc == DMV_OK)
             rc = launch_sum(s->ws, uint64_t(1) << (s->n - bb), s->scalars);
         double total = 0.0;
@@ -364,7 +376,7 @@ static int sample_impl(dmv_state *s, int64_t k, uint64_t seed, const double *u_h
     }
     if (rc == DMV_OK) {
         prof_begin(DMV_KC_SAMPLE);
-        rc = launch_sample(s->dtype, s->amp, s->n, k, seed, d_u, s->cum, s->ws, d_out);
+        rc = launch_sample(s->dtype, s->amp, s->n, s->layout, k, seed, d_u, s->cum, s->ws, d_out);
         prof_end(DMV_KC_SAMPLE, (double)s->bytes);
     }
     if (rc == DMV_OK)
@@ -528,6 +540,7 @@ int dmv_circuit_adjoint(dmv_circuit *c, dmv_state *psi, dmv_state *lam, const do
     CHECK_STATE(lam);
     DMV_CHECK_ARG(psi != lam, "psi and lambda must be different states");
     DMV_CHECK_ARG(psi->n == c->n && lam->n == c->n && psi->dtype == lam->dtype, "state/circuit size mismatch");
+    DMV_CHECK_ARG(same_layout(psi->layout, lam->layout), "psi and lambda differ in element order");
     DMV_TRY(ensure_device());
     int np = 0;
     dmv_circuit_num_parameters(c, &np);
@@ -539,10 +552,10 @@ int dmv_circuit_adjoint(dmv_circuit *c, dmv_state *psi, dmv_state *lam, const do
     const int *dpi, *dpj;
     DMV_TRY(upload_terms(&c->d_terms, &c->cap_terms, c->n, a, n_pairs, pi, pj, b, &da, &db, &dpi, &dpj));
     prof_begin(DMV_KC_REDUCE);
-    DMV_TRY(launch_apply_obs(psi->dtype, psi->amp, lam->amp, c->n, da, n_pairs, dpi, dpj, db, psi->ws, psi->scalars));
+    DMV_TRY(launch_apply_obs(psi->dtype, psi->amp, lam->amp, c->n, psi->layout, da, n_pairs, dpi, dpj, db, psi->ws, psi->scalars));
     prof_end(DMV_KC_REDUCE, 3.0 * (double)psi->bytes);
     // reverse sweep
-    DMV_TRY(prepare_plan(c->adj, c->version, c->n, psi->dtype, true, adjoint_ops(c->gates)));
+    DMV_TRY(prepare_plan(c->adj, c->version, c->n, psi->dtype, true, adjoint_ops(c->gates), psi->layout));
     DevicePlan &dp = c->adj;
     // partials per slot: one per wave of every tile (tile path), one (whole-state path)
     const uint64_t n_tiles = dp.small ? 1 : (uint64_t(1) << (c->n - dp.pp.k)) * ((uint64_t(1) << dp.pp.tlog) / 64);
@@ -568,7 +581,7 @@ int dmv_circuit_adjoint(dmv_circuit *c, dmv_state *psi, dmv_state *lam, const do
     } else {
         for (uint32_t p = 0; p < dp.plan.passes.size(); p++) {
             prof_begin(DMV_KC_ADJOINT);
-            DMV_TRY(launch_tile_adjoint(psi->dtype, dp.pp.rlog, dp.pp.tlog, psi->amp, lam->amp, c->n, dp.d_passes,
+            DMV_TRY(launch_tile_adjoint(psi->dtype, dp.pp.rlog, dp.pp.tlog, psi->amp, lam->amp, c->n, psi->layout, dp.d_passes,
                                         dp.d_phases, dp.d_ops, p, c->d_partials));
             prof_end(DMV_KC_ADJOINT, 4.0 * (double)psi->bytes);
         }
@@ -599,7 +612,7 @@ int dmv_circuit_plan_stats(dmv_circuit *c, int dtype, int *passes, int *phases, 
         return DMV_OK;
     }
     Plan plan;
-    DMV_TRY(build_plan(merged, tile_params(c->n, dtype, false), &plan));
+    DMV_TRY(build_plan(merged, tile_params(c->n, dtype, false, default_layout(c->n, dtype)), &plan));
     if (passes) *passes = (int)plan.passes.size();
     if (phases) *phases = (int)plan.phases.size();
     if (ops) *ops = (int)plan.ops.size();
@@ -611,7 +624,7 @@ int dmv_circuit_plan_ops(dmv_circuit *c, int dtype, int tile_qubits, int *kind, 
                          double *m, int *pass, int capacity, int *count)
 {
     CHECK_CIRCUIT(c);
-    PlanParams pp = tile_params(c->n, dtype, false);
+    PlanParams pp = tile_params(c->n, dtype, false, default_layout(c->n, dtype));
     if (tile_qubits > 0) {
         pp.k = tile_qubits;
         pp.tlog = tile_qubits >= 10 ? 8 : 6;
@@ -676,7 +689,7 @@ int dmv_circuit_plan_json(dmv_circuit *c, int dtype, int adjoint, int tile_qubit
                           int max_gen_per_pass, char *buffer, uint64_t capacity, uint64_t *length)
 {
     CHECK_CIRCUIT(c);
-    PlanParams pp = tile_params(c->n, dtype, adjoint != 0);
+    PlanParams pp = tile_params(c->n, dtype, adjoint != 0, default_layout(c->n, dtype));
     if (tile_qubits > 0) {
         pp.k = tile_qubits;
         pp.tlog = threads_log2 > 0 ? threads_log2 : (tile_qubits >= 10 ? 8 : 6);
@@ -690,7 +703,8 @@ int dmv_circuit_plan_json(dmv_circuit *c, int dtype, int adjoint, int tile_qubit
     std::string o = "{";
     o += "\"n\":" + std::to_string(pp.n) + ",\"k\":" + std::to_string(pp.k) + ",\"rlog\":" + std::to_string(pp.rlog) +
          ",\"tlog\":" + std::to_string(pp.tlog) + ",\"n_slots\":" + std::to_string(plan.n_slots) +
-         ",\"layout\":[" + std::to_string(pp.layout.shift) + "," + std::to_string(pp.layout.blk) + "],\"passes\":[";
+         ",\"layout\":[" + std::to_string(pp.layout.shift) + "," + std::to_string(pp.layout.blk) + "," +
+         std::to_string(pp.layout.fold) + "],\"passes\":[";
     for (size_t i = 0; i < plan.passes.size(); i++) {
         const DevPass &P = plan.passes[i];
         if (i)

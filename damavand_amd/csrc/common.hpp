@@ -157,28 +157,48 @@ inline DMV_HD uint64_t insert_zero_bit(uint64_t x, uint32_t pos)
 inline int popcount64(uint64_t x) { return __builtin_popcountll(x); }
 
 // ------------------------------------------------------- state layout --
-// Amplitude i of a state lives at element i ^ (P(i >> shift) << blk): the
-// state is cut into 2 MiB spans of 1 KiB blocks, and within span s the block
-// index is XORed with P(s), P the GF(2)-linear map sending bit j of the span
-// index to bit (7 j mod 11) of the block index.  In the plain order, the
-// amplitudes a fused pass gathers for a tile over high qubits sit at large
-// power-of-two distances and land on the same HBM channels; the swizzle
-// spreads them (measured on tile-shaped copies at 30 qubits: 3.9-4.1 TB/s
-// plain against 5.1-5.4 TB/s swizzled, with contiguous tiles unchanged).
-// The map is a bijection within every span (no extra memory), keeps 1 KiB
-// blocks contiguous and is linear: elem(a ^ b) = elem(a) ^ elem(b), so
-// kernels combine a tile base, a thread offset and register offsets with
-// XOR.  States of at most one span have the plain layout.
+// Amplitude i of a state lives at element
+//     i ^ (P(s) << blk) ^ (F(s >> 6) << shift),    s = i >> shift.
+// The state is cut into 2 MiB spans of 1 KiB blocks.
+// - Within span s the block index is XORed with P(s), P the GF(2)-linear map
+//   sending bit j of the span index to bit (7 j mod 11) of the block index.
+//   In the plain order, the amplitudes a fused pass gathers for a tile over
+//   high qubits sit at large power-of-two distances and land on the same HBM
+//   channels; the swizzle spreads them (measured on tile-shaped copies at 30
+//   qubits: 3.9-4.1 TB/s plain against 5.1-5.4 TB/s swizzled, with contiguous
+//   tiles unchanged).
+// - The span index itself has its bits 6 and up (spans 128 MiB apart) folded
+//   onto its top `fold` low bits: F sends bit j to bit (6 - fold + j mod fold).
+//   A tile whose high qubits all lie 128 MiB or more apart otherwise touches
+//   only spans equal in their six low bits; on tile-shaped copies at 33
+//   qubits, qubits 23-31 ran at 2.4 TB/s plain and 5.9 TB/s folded.
+// The map is a bijection (no extra memory), keeps 1 KiB blocks contiguous and
+// is linear: elem(a ^ b) = elem(a) ^ elem(b), so kernels combine a tile base,
+// a thread offset and register offsets with XOR.  States of at most one span
+// have the plain layout.
 struct StateLayout {
     uint32_t shift;  // log2 amplitudes per span (0: plain layout)
     uint32_t blk;    // log2 amplitudes per block
+    uint32_t fold;   // low span-index bits that the high ones fold onto (0: none)
+    uint32_t pad;
 };
 
-inline DMV_HD StateLayout state_layout(int n, bool c64)
+constexpr uint32_t kFoldBase = 6;  // span-index bits at and above this fold down
+
+// The layout of an n-qubit state with the given fold width (a state records
+// its own layout when it is created).
+inline StateLayout state_layout(int n, bool c64, uint32_t fold)
 {
     const uint32_t shift = c64 ? 18u : 17u;  // 2 MiB of amplitudes
     const uint32_t blk = c64 ? 7u : 6u;      // 1 KiB
-    return n > (int)shift ? StateLayout{shift, blk} : StateLayout{0u, 0u};
+    if (n <= (int)shift)
+        return StateLayout{0u, 0u, 0u, 0u};
+    return StateLayout{shift, blk, fold > kFoldBase ? kFoldBase : fold, 0u};
+}
+
+inline bool same_layout(const StateLayout &a, const StateLayout &b)
+{
+    return a.shift == b.shift && a.blk == b.blk && a.fold == b.fold;
 }
 
 // P: bit j of the span index -> bit (7 j mod 11) of the block index.
@@ -193,9 +213,26 @@ inline DMV_HD uint64_t span_swizzle(uint64_t s)
     return o;
 }
 
+// F: bit j of h -> bit (kFoldBase - f + j mod f), f in [1, kFoldBase].
+inline DMV_HD uint64_t span_fold(uint64_t h, uint32_t f)
+{
+    uint64_t o = 0;
+    for (int k = 0; k < 16 && h; k++) {
+        o ^= h;
+        h >>= f;
+    }
+    return (o & ((uint64_t(1) << f) - 1)) << (kFoldBase - f);
+}
+
 inline DMV_HD uint64_t elem(uint64_t i, StateLayout L)
 {
-    return L.shift ? i ^ (span_swizzle(i >> L.shift) << L.blk) : i;
+    if (!L.shift)
+        return i;
+    const uint64_t s = i >> L.shift;
+    uint64_t e = i ^ (span_swizzle(s) << L.blk);
+    if (L.fold)
+        e ^= span_fold(s >> kFoldBase, L.fold) << L.shift;
+    return e;
 }
 
 static_assert(kSmallMaxQubitsC128 <= 17 && kSmallMaxQubitsC64 <= 18,
@@ -203,6 +240,6 @@ static_assert(kSmallMaxQubitsC128 <= 17 && kSmallMaxQubitsC64 <= 18,
 static_assert(kMaxQubits - 17 <= 16, "span index wider than the swizzle");
 
 // Layout of private scratch states (batched circuits).
-constexpr StateLayout kPlainLayout{0u, 0u};
+constexpr StateLayout kPlainLayout{0u, 0u, 0u, 0u};
 
 }  // namespace dmv

@@ -122,7 +122,7 @@ static int gate_dispatch(int mode, int ppt, bool nt, void *state, uint64_t items
     }
 }
 
-int launch_gate(int dtype, void *state, int n, const DevOp &op, int ppt, bool nt)
+int launch_gate(int dtype, void *state, int n, StateLayout L, const DevOp &op, int ppt, bool nt)
 {
     DMV_CHECK_ARG(op.target < (uint32_t)n, "target %u out of range", op.target);
     DMV_CHECK_ARG((op.ctrl >> n) == 0 && !((op.ctrl >> op.target) & 1), "bad control mask");
@@ -155,10 +155,8 @@ int launch_gate(int dtype, void *state, int n, const DevOp &op, int ppt, bool nt
     // one work item per changed pair (GEN/SWAP) or amplitude (DIAG/PHASE)
     const uint64_t items = (uint64_t(1) << n) >> P.count;
     if (dtype == DMV_C64)
-        return gate_dispatch<float>(mode, ppt, nt, state, items, op.target, P, ormask, load_mat<float>(op.m),
-                                    state_layout(n, true));
-    return gate_dispatch<double>(mode, ppt, nt, state, items, op.target, P, ormask, load_mat<double>(op.m),
-                                 state_layout(n, false));
+        return gate_dispatch<float>(mode, ppt, nt, state, items, op.target, P, ormask, load_mat<float>(op.m), L);
+    return gate_dispatch<double>(mode, ppt, nt, state, items, op.target, P, ormask, load_mat<double>(op.m), L);
 }
 
 }  // namespace dmv

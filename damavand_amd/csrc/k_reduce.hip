@@ -30,12 +30,11 @@ __global__ void k_init_basis(V *__restrict__ s, uint64_t n_amp, uint64_t basis, 
     }
 }
 
-int launch_init_basis(int dtype, void *state, int n, uint64_t basis)
+int launch_init_basis(int dtype, void *state, int n, StateLayout L, uint64_t basis)
 {
     const uint64_t n_amp = uint64_t(1) << n;
     DMV_CHECK_ARG(basis < n_amp || basis == ~uint64_t(0), "basis index out of range");
     const uint32_t g = grid_for(n_amp, RT * 8);
-    const StateLayout L = state_layout(n, dtype == DMV_C64);
     if (dtype == DMV_C64)
         hipLaunchKernelGGL(k_init_basis<c64>, dim3(g), dim3(RT), 0, stream(), (c64 *)state, n_amp, basis, L);
     else
@@ -164,9 +163,8 @@ __global__ void __launch_bounds__(RT) k_block_norms(const V *__restrict__ s, uin
         partials[blockIdx.x] = acc;
 }
 
-int launch_block_norms(int dtype, const void *state, int n, int bb, double *partials)
+int launch_block_norms(int dtype, const void *state, int n, StateLayout L, int bb, double *partials)
 {
-    const StateLayout L = state_layout(n, dtype == DMV_C64);
     DMV_CHECK_ARG(bb >= 0 && bb <= n && n - bb <= 31 && (L.shift == 0 || bb <= (int)L.shift), "block bits out of range");
     const uint32_t blocks = 1u << (n - bb);
     if (dtype == DMV_C64)
@@ -266,11 +264,10 @@ __global__ void __launch_bounds__(RT) k_expect_z_final(const double *__restrict_
     }
 }
 
-int launch_expect_z(int dtype, const void *state, int n, double *partials, double *out)
+int launch_expect_z(int dtype, const void *state, int n, StateLayout L, double *partials, double *out)
 {
     const int B = n < kBlockBits ? n : kBlockBits;
     const uint64_t nblk = uint64_t(1) << (n - B);
-    const StateLayout L = state_layout(n, dtype == DMV_C64);
     if (dtype == DMV_C64)
         hipLaunchKernelGGL(k_expect_z_part<c64>, dim3((uint32_t)nblk), dim3(RT), 0, stream(), (const c64 *)state,
                            (uint32_t)B, partials, L);
@@ -334,14 +331,13 @@ __global__ void __launch_bounds__(RT) k_obs(const V *__restrict__ s, V *__restri
 }
 
 template <bool WRITE>
-static int run_obs(int dtype, const void *psi, void *lam, int n, const double *a, int n_pairs, const int *pi,
+static int run_obs(int dtype, const void *psi, void *lam, int n, StateLayout L, const double *a, int n_pairs, const int *pi,
                    const int *pj, const double *b, double *partials, double *out)
 {
     DMV_CHECK_ARG(n_pairs >= 0 && n_pairs <= kMaxPairs, "at most %d ZZ terms", kMaxPairs);
     const int B = n < kBlockBits ? n : kBlockBits;
     const uint64_t nblk = uint64_t(1) << (n - B);
     ObsTerms T{a, pi, pj, b, n_pairs};
-    const StateLayout L = state_layout(n, dtype == DMV_C64);
     if (dtype == DMV_C64)
         hipLaunchKernelGGL((k_obs<c64, WRITE>), dim3((uint32_t)nblk), dim3(RT), 0, stream(), (const c64 *)psi,
                            (c64 *)lam, (uint32_t)n, (uint32_t)B, T, partials, L);
@@ -352,16 +348,16 @@ static int run_obs(int dtype, const void *psi, void *lam, int n, const double *a
     return launch_sum(partials, nblk, out);
 }
 
-int launch_expect_zz(int dtype, const void *state, int n, const double *a, int n_pairs, const int *pi,
+int launch_expect_zz(int dtype, const void *state, int n, StateLayout L, const double *a, int n_pairs, const int *pi,
                      const int *pj, const double *b, double *partials, double *out)
 {
-    return run_obs<false>(dtype, state, nullptr, n, a, n_pairs, pi, pj, b, partials, out);
+    return run_obs<false>(dtype, state, nullptr, n, L, a, n_pairs, pi, pj, b, partials, out);
 }
 
-int launch_apply_obs(int dtype, const void *psi, void *lam, int n, const double *a, int n_pairs, const int *pi,
+int launch_apply_obs(int dtype, const void *psi, void *lam, int n, StateLayout L, const double *a, int n_pairs, const int *pi,
                      const int *pj, const double *b, double *partials, double *out)
 {
-    return run_obs<true>(dtype, psi, lam, n, a, n_pairs, pi, pj, b, partials, out);
+    return run_obs<true>(dtype, psi, lam, n, L, a, n_pairs, pi, pj, b, partials, out);
 }
 
 template <class V>
@@ -385,11 +381,10 @@ __global__ void __launch_bounds__(RT) k_inner(const V *__restrict__ a, const V *
     }
 }
 
-int launch_inner(int dtype, const void *a, const void *b, int n, double *partials, double *out2)
+int launch_inner(int dtype, const void *a, const void *b, int n, StateLayout L, double *partials, double *out2)
 {
     const int B = n < kBlockBits ? n : kBlockBits;
     const uint64_t nblk = uint64_t(1) << (n - B);
-    const StateLayout L = state_layout(n, dtype == DMV_C64);
     if (dtype == DMV_C64)
         hipLaunchKernelGGL(k_inner<c64>, dim3((uint32_t)nblk), dim3(RT), 0, stream(), (const c64 *)a, (const c64 *)b,
                            (uint32_t)B, partials, L);
@@ -512,12 +507,12 @@ __global__ void __launch_bounds__(RT) k_sample(const V *__restrict__ s, uint32_t
         out[w] = result;
 }
 
-int launch_sample(int dtype, const void *state, int n, int64_t k, uint64_t seed, const double *u,
+int launch_sample(int dtype, const void *state, int n, StateLayout L, int64_t k, uint64_t seed, const double *u,
                   double *block_cum, double *partials, int64_t *out)
 {
     const int bb = n < kBlockBits ? n : kBlockBits;
     const uint64_t nblk = uint64_t(1) << (n - bb);
-    DMV_TRY(launch_block_norms(dtype, state, n, bb, partials));
+    DMV_TRY(launch_block_norms(dtype, state, n, L, bb, partials));
     hipLaunchKernelGGL(k_block_scan, dim3(1), dim3(RT), 0, stream(), partials, nblk, block_cum);
     DMV_TRY(check_launch("block scan"));
     const uint64_t waves_per_block = RT / 64;
@@ -525,10 +520,10 @@ int launch_sample(int dtype, const void *state, int n, int64_t k, uint64_t seed,
     DMV_CHECK_ARG(blocks <= 0x7FFFFFFFull, "too many samples");
     if (dtype == DMV_C64)
         hipLaunchKernelGGL(k_sample<c64>, dim3((uint32_t)blocks), dim3(RT), 0, stream(), (const c64 *)state,
-                           (uint32_t)bb, nblk, block_cum, k, seed, u, out, state_layout(n, dtype == DMV_C64));
+                           (uint32_t)bb, nblk, block_cum, k, seed, u, out, L);
     else
         hipLaunchKernelGGL(k_sample<c128>, dim3((uint32_t)blocks), dim3(RT), 0, stream(), (const c128 *)state,
-                           (uint32_t)bb, nblk, block_cum, k, seed, u, out, state_layout(n, dtype == DMV_C64));
+                           (uint32_t)bb, nblk, block_cum, k, seed, u, out, L);
     return check_launch("sampler");
 }
 

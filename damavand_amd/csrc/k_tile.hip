@@ -12,17 +12,17 @@
 
 namespace dmv {
 
-int launch_tile(int dtype, int rlog, int tlog, bool nt, void *state, int n, const DevPass *passes,
+int launch_tile(int dtype, int rlog, int tlog, bool nt, void *state, int n, StateLayout L, const DevPass *passes,
                 const DevPhase *phases, const DevOp *ops, uint32_t pass_idx)
 {
     DMV_CHECK_ARG(n >= rlog + tlog && n - (rlog + tlog) <= 31, "tile shape does not fit %d qubits", n);
     int rc = kNotHandled;
     if (dtype == DMV_C128) {
-        rc = launch_tile_c128_main(rlog, tlog, nt, state, n, passes, phases, ops, pass_idx);
+        rc = launch_tile_c128_main(rlog, tlog, nt, state, n, L, passes, phases, ops, pass_idx);
         if (rc == kNotHandled)
-            rc = launch_tile_c128_other(rlog, tlog, nt, state, n, passes, phases, ops, pass_idx);
+            rc = launch_tile_c128_other(rlog, tlog, nt, state, n, L, passes, phases, ops, pass_idx);
     } else {
-        rc = launch_tile_c64(rlog, tlog, nt, state, n, passes, phases, ops, pass_idx);
+        rc = launch_tile_c64(rlog, tlog, nt, state, n, L, passes, phases, ops, pass_idx);
     }
     if (rc != kNotHandled)
         return rc;

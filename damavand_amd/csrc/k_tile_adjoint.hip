@@ -32,32 +32,31 @@ k_tile_adjoint(typename Cplx<R>::T *__restrict__ psi, typename Cplx<R>::T *__res
 }
 
 template <class R, int RLOG, int TLOG>
-static int run_tile_adj(void *psi, void *lam, int n, const DevPass *passes, const DevPhase *phases,
+static int run_tile_adj(void *psi, void *lam, int n, StateLayout L, const DevPass *passes, const DevPhase *phases,
                         const DevOp *ops, uint32_t pass_idx, double *partials)
 {
     using V = typename Cplx<R>::T;
     const uint64_t tiles = uint64_t(1) << (n - (RLOG + TLOG));
     hipLaunchKernelGGL((k_tile_adjoint<R, RLOG, TLOG>), dim3((uint32_t)tiles), dim3(1 << TLOG), 0, stream(),
-                       (V *)psi, (V *)lam, passes, phases, ops, pass_idx, partials,
-                       state_layout(n, sizeof(R) == 4));
+                       (V *)psi, (V *)lam, passes, phases, ops, pass_idx, partials, L);
     return check_launch("adjoint tile pass");
 }
 
-int launch_tile_adjoint(int dtype, int rlog, int tlog, void *psi, void *lam, int n, const DevPass *passes,
+int launch_tile_adjoint(int dtype, int rlog, int tlog, void *psi, void *lam, int n, StateLayout L, const DevPass *passes,
                         const DevPhase *phases, const DevOp *ops, uint32_t pass_idx, double *partials)
 {
     DMV_CHECK_ARG(n >= rlog + tlog && n - (rlog + tlog) <= 31, "tile shape does not fit %d qubits", n);
     if (dtype == DMV_C128) {
-        if (tlog == 8 && rlog == 4) return run_tile_adj<double, 4, 8>(psi, lam, n, passes, phases, ops, pass_idx, partials);
-        if (tlog == 8 && rlog == 3) return run_tile_adj<double, 3, 8>(psi, lam, n, passes, phases, ops, pass_idx, partials);
-        if (tlog == 8 && rlog == 2) return run_tile_adj<double, 2, 8>(psi, lam, n, passes, phases, ops, pass_idx, partials);
-        if (tlog == 6 && rlog == 3) return run_tile_adj<double, 3, 6>(psi, lam, n, passes, phases, ops, pass_idx, partials);
+        if (tlog == 8 && rlog == 4) return run_tile_adj<double, 4, 8>(psi, lam, n, L, passes, phases, ops, pass_idx, partials);
+        if (tlog == 8 && rlog == 3) return run_tile_adj<double, 3, 8>(psi, lam, n, L, passes, phases, ops, pass_idx, partials);
+        if (tlog == 8 && rlog == 2) return run_tile_adj<double, 2, 8>(psi, lam, n, L, passes, phases, ops, pass_idx, partials);
+        if (tlog == 6 && rlog == 3) return run_tile_adj<double, 3, 6>(psi, lam, n, L, passes, phases, ops, pass_idx, partials);
     } else {
-        if (tlog == 8 && rlog == 5) return run_tile_adj<float, 5, 8>(psi, lam, n, passes, phases, ops, pass_idx, partials);
-        if (tlog == 8 && rlog == 4) return run_tile_adj<float, 4, 8>(psi, lam, n, passes, phases, ops, pass_idx, partials);
-        if (tlog == 8 && rlog == 3) return run_tile_adj<float, 3, 8>(psi, lam, n, passes, phases, ops, pass_idx, partials);
-        if (tlog == 6 && rlog == 3) return run_tile_adj<float, 3, 6>(psi, lam, n, passes, phases, ops, pass_idx, partials);
-        if (tlog == 6 && rlog == 4) return run_tile_adj<float, 4, 6>(psi, lam, n, passes, phases, ops, pass_idx, partials);
+        if (tlog == 8 && rlog == 5) return run_tile_adj<float, 5, 8>(psi, lam, n, L, passes, phases, ops, pass_idx, partials);
+        if (tlog == 8 && rlog == 4) return run_tile_adj<float, 4, 8>(psi, lam, n, L, passes, phases, ops, pass_idx, partials);
+        if (tlog == 8 && rlog == 3) return run_tile_adj<float, 3, 8>(psi, lam, n, L, passes, phases, ops, pass_idx, partials);
+        if (tlog == 6 && rlog == 3) return run_tile_adj<float, 3, 6>(psi, lam, n, L, passes, phases, ops, pass_idx, partials);
+        if (tlog == 6 && rlog == 4) return run_tile_adj<float, 4, 6>(psi, lam, n, L, passes, phases, ops, pass_idx, partials);
     }
     set_error("no adjoint tile kernel for dtype %d rlog %d tlog %d", dtype, rlog, tlog);
     return DMV_ERR_ARG;

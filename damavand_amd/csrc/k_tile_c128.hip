@@ -3,11 +3,11 @@
 
 namespace dmv {
 
-int launch_tile_c128_main(int rlog, int tlog, bool nt, void *state, int n, const DevPass *passes, const DevPhase *phases,
+int launch_tile_c128_main(int rlog, int tlog, bool nt, void *state, int n, StateLayout L, const DevPass *passes, const DevPhase *phases,
                           const DevOp *ops, uint32_t pass_idx)
 {
-    if (tlog == 9 && rlog == 4) return run_tile<double, 4, 9>(nt, state, n, passes, phases, ops, pass_idx);
-    if (tlog == 8 && rlog == 4) return run_tile<double, 4, 8>(nt, state, n, passes, phases, ops, pass_idx);
+    if (tlog == 9 && rlog == 4) return run_tile<double, 4, 9>(nt, state, n, L, passes, phases, ops, pass_idx);
+    if (tlog == 8 && rlog == 4) return run_tile<double, 4, 8>(nt, state, n, L, passes, phases, ops, pass_idx);
     return kNotHandled;
 }
 

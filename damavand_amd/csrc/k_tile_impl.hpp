@@ -38,12 +38,11 @@ k_tile_pipe(typename Cplx<R>::T *__restrict__ state, const DevPass *__restrict__
 
 // ------------------------------------------------------------- launchers --
 template <class R, int RLOG, int TLOG>
-static int run_tile(bool nt, void *state, int n, const DevPass *passes, const DevPhase *phases, const DevOp *ops,
+static int run_tile(bool nt, void *state, int n, StateLayout L, const DevPass *passes, const DevPhase *phases, const DevOp *ops,
                     uint32_t pass_idx)
 {
     using V = typename Cplx<R>::T;
     const uint64_t tiles = uint64_t(1) << (n - (RLOG + TLOG));
-    const StateLayout L = state_layout(n, sizeof(R) == 4);
     const int64_t per_cu = tune_value("tile_wgs_per_cu", 2);
     const int64_t forced = tune_value("tile_grid", 0);  // explicit persistent grid (tests)
     const uint64_t grid64 = forced > 0 ? (uint64_t)forced : (uint64_t)device_cus() * (uint64_t)(per_cu > 0 ? per_cu : 1);
@@ -71,11 +70,11 @@ static int run_tile(bool nt, void *state, int n, const DevPass *passes, const De
 // Shape-specific launchers (one translation unit each); they return
 // kNotHandled when the shape is not theirs.
 constexpr int kNotHandled = -1;
-int launch_tile_c128_main(int rlog, int tlog, bool nt, void *state, int n, const DevPass *passes,
+int launch_tile_c128_main(int rlog, int tlog, bool nt, void *state, int n, StateLayout L, const DevPass *passes,
                           const DevPhase *phases, const DevOp *ops, uint32_t pass_idx);
-int launch_tile_c128_other(int rlog, int tlog, bool nt, void *state, int n, const DevPass *passes,
+int launch_tile_c128_other(int rlog, int tlog, bool nt, void *state, int n, StateLayout L, const DevPass *passes,
                            const DevPhase *phases, const DevOp *ops, uint32_t pass_idx);
-int launch_tile_c64(int rlog, int tlog, bool nt, void *state, int n, const DevPass *passes,
+int launch_tile_c64(int rlog, int tlog, bool nt, void *state, int n, StateLayout L, const DevPass *passes,
                     const DevPhase *phases, const DevOp *ops, uint32_t pass_idx);
 
 }  // namespace dmv

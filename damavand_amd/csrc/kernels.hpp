@@ -8,12 +8,13 @@
 
 namespace dmv {
 
-// One fused pass of a tile plan over a state of n qubits.
-int launch_tile(int dtype, int rlog, int tlog, bool nontemporal, void *state, int n,
+// One fused pass of a tile plan over a state of n qubits in layout L (every
+// state launcher below takes the state's layout L).
+int launch_tile(int dtype, int rlog, int tlog, bool nontemporal, void *state, int n, StateLayout L,
                 const DevPass *passes, const DevPhase *phases, const DevOp *ops, uint32_t pass_idx);
 // Reverse-sweep tile pass over psi and lambda; IP ops write one partial per
 // (slot, tile) into partials[slot * n_tiles + tile].
-int launch_tile_adjoint(int dtype, int rlog, int tlog, void *psi, void *lam, int n,
+int launch_tile_adjoint(int dtype, int rlog, int tlog, void *psi, void *lam, int n, StateLayout L,
                         const DevPass *passes, const DevPhase *phases, const DevOp *ops,
                         uint32_t pass_idx, double *partials);
 // Every op of a list applied to a whole state held in one workgroup's LDS.
@@ -52,10 +53,10 @@ int batch_tile_shape(int dtype, int *rlog, int *tlog);
 int launch_batch(int dtype, const BatchLaunch &L);
 
 // One op as its own pass over the amplitudes it changes (global qubits).
-int launch_gate(int dtype, void *state, int n, const DevOp &op, int pairs_per_thread, bool nontemporal);
+int launch_gate(int dtype, void *state, int n, StateLayout L, const DevOp &op, int pairs_per_thread, bool nontemporal);
 
 // --- states, reductions, sampling ---
-int launch_init_basis(int dtype, void *state, int n, uint64_t basis);
+int launch_init_basis(int dtype, void *state, int n, StateLayout L, uint64_t basis);
 int launch_copy(void *dst, const void *src, uint64_t bytes);
 // Interleaved amplitudes [begin, begin+count) -> split re/im doubles (device).
 int launch_split(int dtype, const void *state, StateLayout L, uint64_t begin, uint64_t count, double *re,
@@ -64,19 +65,19 @@ int launch_join(int dtype, void *state, StateLayout L, uint64_t begin, uint64_t 
                 const double *im);
 int launch_probs(int dtype, const void *state, StateLayout L, uint64_t begin, uint64_t count, double *out);
 // Partial sums of |a|^2 over blocks of 2^block_bits amplitudes.
-int launch_block_norms(int dtype, const void *state, int n, int block_bits, double *partials);
+int launch_block_norms(int dtype, const void *state, int n, StateLayout L, int block_bits, double *partials);
 // Deterministic single-workgroup sum of `count` doubles into *out (device).
 int launch_sum(const double *in, uint64_t count, double *out);
-int launch_expect_z(int dtype, const void *state, int n, double *partials, double *out);
-int launch_expect_zz(int dtype, const void *state, int n, const double *a, int n_pairs,
+int launch_expect_z(int dtype, const void *state, int n, StateLayout L, double *partials, double *out);
+int launch_expect_zz(int dtype, const void *state, int n, StateLayout L, const double *a, int n_pairs,
                      const int *pi, const int *pj, const double *b, double *partials, double *out);
 // lambda = h * psi for the Z/ZZ observable, plus <H> into *out.
-int launch_apply_obs(int dtype, const void *psi, void *lam, int n, const double *a, int n_pairs,
+int launch_apply_obs(int dtype, const void *psi, void *lam, int n, StateLayout L, const double *a, int n_pairs,
                      const int *pi, const int *pj, const double *b, double *partials, double *out);
-int launch_inner(int dtype, const void *a, const void *b, int n, double *partials, double *out2);
+int launch_inner(int dtype, const void *a, const void *b, int n, StateLayout L, double *partials, double *out2);
 // Sum each slot's partials: out[s] = sum_t partials[s * n_tiles + t].
 int launch_slot_sums(const double *partials, int n_slots, uint64_t n_tiles, double *out);
-int launch_sample(int dtype, const void *state, int n, int64_t k, uint64_t seed, const double *u,
+int launch_sample(int dtype, const void *state, int n, StateLayout L, int64_t k, uint64_t seed, const double *u,
                   double *block_cum, double *partials, int64_t *out);
 
 }  // namespace dmv

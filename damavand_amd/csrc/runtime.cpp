@@ -249,6 +249,23 @@ int64_t tune_value(const char *name, int64_t dflt)
 
 bool tune_nontemporal() { return tune_value("nontemporal", 0) != 0; }
 
+// Layout of a new state.  The span index is folded on states of 2^15 spans
+// and more (32 qubits in complex128): there, tiles over the highest qubits
+// otherwise run at 2.4 TB/s (measured: 33 qubits, qubits 23-31), while at 30
+// qubits the fold measured about 1% slower on the layered workload.
+// Tunables: page_fold_span_bits (span-index width from which to fold),
+// page_fold (fold width).
+StateLayout default_layout(int n, int dtype)
+{
+    const bool c64 = dtype == DMV_C64;
+    const StateLayout plain = state_layout(n, c64, 0);
+    const int64_t from = tune_value("page_fold_span_bits", 15);
+    const int64_t width = tune_value("page_fold", kFoldBase);
+    if (!plain.shift || n - (int)plain.shift < from || width <= 0)
+        return plain;
+    return state_layout(n, c64, (uint32_t)width);
+}
+
 // Single-gate kernels: one pair per thread; nontemporal loads/stores unless
 // the target is one of the lowest qubits, where the two streams share lines
 // (measured best on MI355X at 30 qubits).
@@ -263,11 +280,11 @@ bool gate_nontemporal(uint32_t target, int dtype)
 }
 
 // ------------------------------------------------------------------- plans --
-PlanParams tile_params(int n, int dtype, bool adjoint)
+PlanParams tile_params(int n, int dtype, bool adjoint, StateLayout layout)
 {
     PlanParams pp;
     pp.n = n;
-    pp.layout = state_layout(n, dtype == DMV_C64);
+    pp.layout = layout;
     pp.low_bits = coalesce_bits(dtype);
     // wider contiguous runs per lane group (tunable; fewer free tile qubits)
     const int64_t runs = tune_value("coalesce_qubits", 0);
@@ -326,13 +343,15 @@ template <class T> static int upload(T *&dptr, size_t &cap, const std::vector<T>
     return copy_h2d(dptr, v.data(), v.size() * sizeof(T));
 }
 
-int prepare_plan(DevicePlan &dp, uint64_t version, int n, int dtype, bool adjoint, const std::vector<HostOp> &ops)
+int prepare_plan(DevicePlan &dp, uint64_t version, int n, int dtype, bool adjoint, const std::vector<HostOp> &ops,
+                 StateLayout layout)
 {
     const int small_cap = (dtype == DMV_C64 ? kSmallMaxQubitsC64 : kSmallMaxQubitsC128) - (adjoint ? 1 : 0);
     const bool small = n <= small_cap;
-    PlanParams pp = small ? PlanParams{} : tile_params(n, dtype, adjoint);
+    PlanParams pp = small ? PlanParams{} : tile_params(n, dtype, adjoint, layout);
     const bool same_shape = dp.valid && dp.dtype == dtype && dp.small == small &&
-                            (small || (dp.pp.k == pp.k && dp.pp.tlog == pp.tlog && dp.pp.max_gen_per_pass == pp.max_gen_per_pass));
+                            (small || (dp.pp.k == pp.k && dp.pp.tlog == pp.tlog && dp.pp.max_gen_per_pass == pp.max_gen_per_pass &&
+                                       same_layout(dp.pp.layout, pp.layout)));
     if (same_shape && dp.version == version && dp.epoch == tune_epoch())
         return DMV_OK;
     std::vector<HostOp> merged = merge_ops(ops);
@@ -381,9 +400,14 @@ int state_create(int n, int dtype, dmv_state **out)
     s->dtype = dtype;
     s->bytes = (uint64_t(1) << n) * (uint64_t)amp_bytes(dtype);
     s->alloc_bytes = s->bytes;  // the swizzled layout is a permutation within the plain size
+    s->layout = default_layout(n, dtype);
     s->ws_doubles = ws_doubles_for(n);
     const uint64_t nblk = s->ws_doubles;
-    int rc = alloc(&s->amp, s->alloc_bytes);
+    int rc = DMV_ERR_MEMORY;
+    if (tune_value("spread_states", 1) && spread_alloc(s->alloc_bytes, &s->spread) == DMV_OK)
+        s->amp = s->spread.va, rc = DMV_OK;
+    if (rc != DMV_OK)
+        rc = alloc(&s->amp, s->alloc_bytes);
     if (rc == DMV_OK)
         rc = alloc((void **)&s->ws, s->ws_doubles * sizeof(double));
     if (rc == DMV_OK)
@@ -404,7 +428,10 @@ void state_destroy(dmv_state *s)
 {
     if (!s)
         return;
-    release(s->amp);
+    if (s->spread.va)
+        spread_free(&s->spread);
+    else
+        release(s->amp);
     release(s->ws);
     release(s->cum);
     release(s->scalars);
@@ -415,7 +442,7 @@ int state_reset(dmv_state *s, uint64_t basis, bool zero)
 {
     DMV_TRY(ensure_device());
     prof_begin(DMV_KC_REDUCE);
-    DMV_TRY(launch_init_basis(s->dtype, s->amp, s->n, zero ? ~uint64_t(0) : basis));
+    DMV_TRY(launch_init_basis(s->dtype, s->amp, s->n, s->layout, zero ? ~uint64_t(0) : basis));
     prof_end(DMV_KC_REDUCE, (double)s->bytes);
     return DMV_OK;
 }
@@ -445,7 +472,7 @@ int state_get(dmv_state *s, double *re, double *im)
     double *dre, *dim;
     DMV_TRY(staging(&dre, &dim, chunk));
     for (uint64_t b = 0; b < n_amp; b += chunk) {
-        DMV_TRY(launch_split(s->dtype, s->amp, state_layout(s->n, s->dtype == DMV_C64), b, chunk, dre, dim));
+        DMV_TRY(launch_split(s->dtype, s->amp, s->layout, b, chunk, dre, dim));
         if (re)
             DMV_TRY(copy_d2h(re + b, dre, chunk * sizeof(double)));
         if (im)
@@ -464,7 +491,7 @@ int state_set(dmv_state *s, const double *re, const double *im)
     for (uint64_t b = 0; b < n_amp; b += chunk) {
         DMV_TRY(copy_h2d(dre, re + b, chunk * sizeof(double)));
         DMV_TRY(copy_h2d(dim, im + b, chunk * sizeof(double)));
-        DMV_TRY(launch_join(s->dtype, s->amp, state_layout(s->n, s->dtype == DMV_C64), b, chunk, dre, dim));
+        DMV_TRY(launch_join(s->dtype, s->amp, s->layout, b, chunk, dre, dim));
     }
     return synchronize();
 }
@@ -478,7 +505,7 @@ int state_probs(dmv_state *s, double *host)
     DMV_TRY(staging(&dre, &dim, chunk));
     for (uint64_t b = 0; b < n_amp; b += chunk) {
         prof_begin(DMV_KC_REDUCE);
-        DMV_TRY(launch_probs(s->dtype, s->amp, state_layout(s->n, s->dtype == DMV_C64), b, chunk, dre));
+        DMV_TRY(launch_probs(s->dtype, s->amp, s->layout, b, chunk, dre));
         prof_end(DMV_KC_REDUCE, (double)chunk * (amp_bytes(s->dtype) + 8));
         DMV_TRY(copy_d2h(host + b, dre, chunk * sizeof(double)));
     }
@@ -497,7 +524,7 @@ static int forward_passes(DevicePlan &dp, dmv_state *s)
     const bool nt = tune_nontemporal();
     for (uint32_t p = 0; p < dp.plan.passes.size(); p++) {
         prof_begin(DMV_KC_TILE);
-        DMV_TRY(launch_tile(s->dtype, dp.pp.rlog, dp.pp.tlog, nt, s->amp, s->n, dp.d_passes, dp.d_phases, dp.d_ops, p));
+        DMV_TRY(launch_tile(s->dtype, dp.pp.rlog, dp.pp.tlog, nt, s->amp, s->n, s->layout, dp.d_passes, dp.d_phases, dp.d_ops, p));
         prof_end(DMV_KC_TILE, 2.0 * (double)s->bytes);
     }
     return DMV_OK;
@@ -564,13 +591,13 @@ int circuit_forward(dmv_circuit *c, dmv_state *s, int mode)
             const bool phase_only = h.kind == OP_DIAG && h.m[0] == 1.0 && h.m[1] == 0.0;
             const double touched = (double)(n_amp >> (nc + (phase_only ? 1 : 0)));
             prof_begin(DMV_KC_GATE);
-            DMV_TRY(launch_gate(s->dtype, s->amp, s->n, op, ppt, nt));
+            DMV_TRY(launch_gate(s->dtype, s->amp, s->n, s->layout, op, ppt, nt));
             prof_end(DMV_KC_GATE, 2.0 * touched * amp_bytes(s->dtype));
         }
         return DMV_OK;
     }
     DMV_CHECK_ARG(mode == DMV_FORWARD_FUSED, "unknown forward mode %d", mode);
-    DMV_TRY(prepare_plan(c->fwd, c->version, c->n, s->dtype, false, forward_ops(c->gates)));
+    DMV_TRY(prepare_plan(c->fwd, c->version, c->n, s->dtype, false, forward_ops(c->gates), s->layout));
     if (tune_value("graph", 0) && !c->fwd.small)
         return forward_graph(c->fwd, s);
     return forward_passes(c->fwd, s);

@@ -8,12 +8,28 @@
 #include "common.hpp"
 #include "plan.hpp"
 
+namespace dmv {
+
+// A device buffer made of kSpreadPieces physical allocations mapped back to
+// back into one reserved address range (spread_alloc).
+constexpr int kSpreadPieces = 8;
+struct SpreadBuffer {
+    void *va = nullptr;
+    uint64_t bytes = 0;
+    uint64_t piece = 0;
+    hipMemGenericAllocationHandle_t h[kSpreadPieces] = {};
+};
+
+}  // namespace dmv
+
 struct dmv_state {
     int n = 0;
     int dtype = DMV_C128;
-    void *amp = nullptr;          // 2^n interleaved amplitudes in the layout of state_layout(n, dtype)
+    void *amp = nullptr;          // 2^n interleaved amplitudes, element order `layout`
     uint64_t bytes = 0;           // 2^n amplitudes' worth (algorithmic size)
     uint64_t alloc_bytes = 0;     // allocation size
+    dmv::SpreadBuffer spread;     // set when amp is a spread buffer (else amp came from alloc)
+    dmv::StateLayout layout{};    // element order of amp, fixed at creation
     double *ws = nullptr;         // reduction partials
     uint64_t ws_doubles = 0;
     double *cum = nullptr;        // sampler block prefix
@@ -64,16 +80,25 @@ int ensure_device();              // initialises the context, selects the device
 int synchronize();
 int alloc(void **p, uint64_t bytes);
 void release(void *p);
+// Large state buffers whose physical pieces lie kSpreadStride apart (see
+// spread_alloc.cpp); DMV_ERR_MEMORY when the device cannot do it, with no
+// error left set, so that the caller falls back to alloc.
+int spread_alloc(uint64_t bytes, SpreadBuffer *out);
+void spread_free(SpreadBuffer *b);
 int copy_h2d(void *dst, const void *src, uint64_t bytes);
 int copy_d2h(void *dst, const void *src, uint64_t bytes);
 int64_t tune_value(const char *name, int64_t dflt);
 uint64_t tune_epoch();
 
-// Tile shape for a forward (adjoint = false) or reverse sweep on n qubits.
-PlanParams tile_params(int n, int dtype, bool adjoint);
-// Builds (if stale) and uploads the plan for `ops` into `dp`.
+// Layout a new n-qubit state gets (tunables page_fold_span_bits, page_fold).
+StateLayout default_layout(int n, int dtype);
+// Tile shape for a forward (adjoint = false) or reverse sweep on n qubits
+// over states of the given layout.
+PlanParams tile_params(int n, int dtype, bool adjoint, StateLayout layout);
+// Builds (if stale) and uploads the plan for `ops` into `dp`, for states of
+// the given layout.
 int prepare_plan(DevicePlan &dp, uint64_t version, int n, int dtype, bool adjoint,
-                 const std::vector<HostOp> &ops);
+                 const std::vector<HostOp> &ops, StateLayout layout);
 void free_plan(DevicePlan &dp);
 
 int state_create(int n, int dtype, dmv_state **out);

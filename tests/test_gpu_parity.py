@@ -411,18 +411,86 @@ def test_hipgraph_replays(native):
         N.tune_set("graph", 0)
 
 
-@pytest.mark.parametrize("dtype,n", [("complex128", 19), ("complex64", 20)])
-def test_state_layout_swizzled(native, dtype, n):
-    """States past 2 MiB of amplitudes have the swizzled order: the reported
-    layout and size match, and loading, fused and per-gate passes over high
-    qubits, reductions and readback all agree with the oracle."""
-    say(f"case state layout {dtype} n={n}")
+def _read_device(ptr, nbytes):
+    """Raw bytes of device memory, copied with the HIP runtime instance the
+    library itself loaded (found in this process's mappings)."""
+    N.lib()
+    path = next(line.split()[-1] for line in open("/proc/self/maps")
+                if "libamdhip64.so" in line and "/torch/" not in line)
+    hip = ctypes.CDLL(path)
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    out = np.empty(nbytes, dtype=np.uint8)
+    N.synchronize()
+    assert hip.hipMemcpy(out.ctypes.data, ptr, nbytes, 2) == 0  # hipMemcpyDeviceToHost
+    return out
+
+
+def test_states_of_other_layouts(native):
+    """A circuit runs on states created under different layout settings (its
+    plan is rebuilt for each), and operations that pair two states reject
+    states whose element orders differ."""
+    n = 25
+    say(f"case states of other layouts n={n}")
+    N.tune_set("page_fold_span_bits", 0)
+    try:
+        folded = StateVector(n)
+    finally:
+        N.tune_set("page_fold_span_bits", 15)
+    plain = StateVector(n)
+    gates = [(wl.GATE_H, -1, q, ()) for q in range(n)] + [(wl.GATE_CNOT, 3, n - 1, ()),
+                                                         (wl.GATE_RY, -1, n - 2, (0.4,)), (wl.GATE_CZ, n - 1, 1, ())]
+    c = build_circuit(n, gates)
+    got = []
+    for sv in (folded, plain, folded):
+        sv.reset(0)
+        N.call("dmv_circuit_forward", c._c.h, sv.handle, 0)
+        got.append(sv.amplitudes())
+    ref = oracle.run_gates(n, gates).amp
+    for g in got:
+        assert np.max(np.abs(g - ref)) <= TOL
+    out = np.zeros(2)
+    with pytest.raises(N.DamavandError, match="element order"):
+        N.call("dmv_inner_product", folded.handle, plain.handle, N.dptr(out))
+    with pytest.raises(N.DamavandError, match="element order"):
+        N.call("dmv_state_copy", plain.handle, folded.handle)
+
+
+@pytest.mark.parametrize("dtype,n,spread,fold", [("complex128", 19, False, False), ("complex64", 20, False, False),
+                                                 ("complex128", 25, False, True), ("complex64", 26, False, True),
+                                                 ("complex128", 25, True, True), ("complex64", 26, True, False)])
+def test_state_layout_swizzled(native, dtype, n, spread, fold):
+    """States past 2 MiB of amplitudes have the swizzled order: the raw device
+    buffer holds amplitude i at the element the header's formula names, and
+    loading, fused and per-gate passes over high qubits, reductions and
+    readback all agree with the oracle.  With fold, the span-index fold (by
+    default from 32 qubits on) is switched on for these sizes, where the span
+    index is wide enough for it to act.  With spread, the size threshold of
+    spread buffers is lowered so that these states are eight physical pieces
+    mapped back to back."""
+    N.tune_set("spread_min_bytes", (64 << 20) if spread else (1 << 62))
+    N.tune_set("page_fold_span_bits", 0 if fold else 99)
+    try:
+        _check_state_layout(dtype, n, spread, fold)
+    finally:
+        N.tune_set("spread_min_bytes", 2 << 30)
+        N.tune_set("page_fold_span_bits", 15)
+
+
+def _check_state_layout(dtype, n, spread, fold):
+    from tests.tile_emulator import element_map
+
+    say(f"case state layout {dtype} n={n} spread={spread} fold={fold}")
     c64 = dtype == "complex64"
     tol = 1e-5 if c64 else TOL
     sv = StateVector(n, dtype)
-    shift, blk = ctypes.c_int(0), ctypes.c_int(0)
-    N.call("dmv_state_layout", sv.handle, ctypes.byref(shift), ctypes.byref(blk))
+    pieces = ctypes.c_int(0)
+    N.call("dmv_state_pieces", sv.handle, ctypes.byref(pieces))
+    assert pieces.value == (8 if spread else 1)
+    fold_on = fold
+    shift, blk, fold = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(-1)
+    N.call("dmv_state_layout", sv.handle, ctypes.byref(shift), ctypes.byref(blk), ctypes.byref(fold))
     assert (shift.value, blk.value) == ((18, 7) if c64 else (17, 6))
+    assert fold.value == (6 if fold_on else 0)
     ptr, nbytes = ctypes.c_void_p(), ctypes.c_uint64(0)
     N.call("dmv_state_device_ptr", sv.handle, ctypes.byref(ptr), ctypes.byref(nbytes))
     assert nbytes.value == (1 << n) * (8 if c64 else 16)
@@ -431,6 +499,10 @@ def test_state_layout_swizzled(native, dtype, n):
     amp /= np.linalg.norm(amp)
     sv.load(amp)
     np.testing.assert_allclose(sv.amplitudes(), amp, atol=1e-7 if c64 else 0)
+    raw = _read_device(ptr.value, nbytes.value).view(np.float32 if c64 else np.float64)
+    raw = raw[0::2] + 1j * raw[1::2]
+    want = amp.astype(np.complex64) if c64 else amp
+    assert np.array_equal(raw[element_map(np.arange(1 << n), shift.value, blk.value, fold.value)], want)
     gates = [(wl.GATE_H, -1, n - 1, ()), (wl.GATE_CNOT, n - 1, n - 2, ()), (wl.GATE_RX, -1, n - 3, (0.7,)),
              (wl.GATE_CNOT, 0, n - 1, ()), (wl.GATE_H, -1, 1, ()), (wl.GATE_RZ, -1, n - 1, (1.1,))]
     ref = oracle.OracleState(n)
@@ -439,6 +511,8 @@ def test_state_layout_swizzled(native, dtype, n):
         ref.apply_gate(kind, t, ctl, p)
     for per_gate in (False, True):
         c = build_circuit(n, gates, dtype)
+        N.call("dmv_state_pieces", c.state.handle, ctypes.byref(pieces))
+        assert pieces.value == (8 if spread else 1)
         c.state.load(amp)
         c.forward(per_gate=per_gate)
         err = np.max(np.abs(c.retrieve_amplitudes_on_host() - ref.amp))

@@ -9,7 +9,7 @@ import oracle
 from damavand_amd import _native as N
 from damavand_amd import workloads as wl
 from tests.libhelpers import NativeCircuit
-from tests.tile_emulator import TileEmulator, plan_json
+from tests.tile_emulator import TileEmulator, element_map, plan_json
 
 
 def random_gates(rng, n, count):
@@ -145,8 +145,29 @@ def test_gapped_layout_emulation():
     gates = random_gates(rng, n, 120) + [(wl.GATE_H, -1, q, ()) for q in range(12, n)]
     nc = NativeCircuit(n, gates)
     plan = plan_json(nc.h, tile_qubits=12, tlog=8, max_gen=8)
-    assert plan["layout"] == [17, 6]
+    assert plan["layout"] == [17, 6, 0]
     assert max(max(P["greg_ld"]) for P in plan["passes"]) >= (1 << 17)
+    got = TileEmulator(plan).forward(zero_state(n))
+    np.testing.assert_allclose(got, oracle.run_gates(n, gates).amp, atol=1e-12)
+
+
+def test_page_fold_emulation():
+    """A state of more than 64 spans, where the span index folds: passes over
+    the highest qubits (register offsets that cross the fold) address every
+    element once and match the oracle."""
+    n = 24
+    gates = [(wl.GATE_H, -1, q, ()) for q in range(15, n)] + [(wl.GATE_CNOT, n - 1, 3, ()),
+                                                               (wl.GATE_RY, -1, n - 2, (0.3,)),
+                                                               (wl.GATE_CNOT, 2, n - 1, ())]
+    nc = NativeCircuit(n, gates)
+    N.tune_set("page_fold_span_bits", 0)  # fold from the smallest states on (default: 32 qubits and up)
+    try:
+        plan = plan_json(nc.h, tile_qubits=12, tlog=8, max_gen=8)
+    finally:
+        N.tune_set("page_fold_span_bits", 15)
+    assert plan["layout"] == [17, 6, 6]
+    # some register offset has a span part of 64 or more, so the fold acts on it
+    assert any((g >> 17) >= 64 for P in plan["passes"] for g in P["greg_ld"])
     got = TileEmulator(plan).forward(zero_state(n))
     np.testing.assert_allclose(got, oracle.run_gates(n, gates).amp, atol=1e-12)
 
@@ -167,3 +188,25 @@ def test_wider_coalescing_runs_emulation(runs):
         assert P["gthr_ld"][:runs] == list(range(runs)) and P["gthr_st"][:runs] == list(range(runs))
     got = TileEmulator(plan).forward(zero_state(n))
     np.testing.assert_allclose(got, oracle.run_gates(n, gates).amp, atol=1e-12)
+
+
+@pytest.mark.parametrize("shift,blk,fold", [(17, 6, 6), (18, 7, 6), (17, 6, 2), (17, 6, 0)])
+def test_element_map_properties(shift, blk, fold):
+    """The documented element map is a bijection, XOR-linear, and keeps 1 KiB
+    blocks contiguous; with the fold, a tile over qubits 23-31 (33 qubits)
+    touches spans of every value of the folded low span bits."""
+    n = 24
+    i = np.arange(1 << n, dtype=np.int64)
+    e = element_map(i, shift, blk, fold)
+    assert np.array_equal(np.sort(e), i)
+    assert np.array_equal(e ^ (i & ((1 << blk) - 1)), element_map(i & ~np.int64((1 << blk) - 1), shift, blk, fold))
+    rng = np.random.default_rng(shift + fold)
+    a = rng.integers(0, 1 << 33, size=4096, dtype=np.int64)
+    b = rng.integers(0, 1 << 33, size=4096, dtype=np.int64)
+    assert np.array_equal(element_map(a ^ b, shift, blk, fold),
+                          element_map(a, shift, blk, fold) ^ element_map(b, shift, blk, fold))
+    hi = np.zeros(512, dtype=np.int64)
+    for j in range(9):
+        hi |= ((np.arange(512) >> j) & 1) << (23 + j + (shift - 17))
+    low_span = (element_map(hi, shift, blk, fold) >> shift) & 63
+    assert len(np.unique(low_span)) == (1 << fold if fold else 1)

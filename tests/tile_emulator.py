@@ -48,6 +48,25 @@ def highest_bit(m):
     return 1
 
 
+def element_map(i, shift, blk, fold):
+    """Element of amplitude i in the device layout, as include/damavand_gpu.h
+    states it: with s = i >> shift, i ^ (P(s) << blk) ^ (F(s >> 6) << shift),
+    P: bit j -> bit (7 j mod 11), F: bit j -> bit (6 - fold + j mod fold)."""
+    i = np.asarray(i, dtype=np.int64)
+    if not shift:
+        return i
+    span = i >> shift
+    p = np.zeros_like(span)
+    for j in range(16):
+        p ^= ((span >> j) & 1) << ((7 * j) % 11)
+    f = np.zeros_like(span)
+    if fold:
+        h = span >> 6
+        for j in range(16):
+            f ^= ((h >> j) & 1) << (6 - fold + j % fold)
+    return i ^ (p << blk) ^ (f << shift)
+
+
 class TileEmulator:
     def __init__(self, plan, elem=16):
         self.plan = plan
@@ -162,16 +181,7 @@ class TileEmulator:
             v[:, tm, q] = a10 * x + a11 * y
 
     def element(self, i):
-        # element of amplitude i in the device layout: within each span of 2^shift amplitudes the
-        # block index (blocks of 2^blk) is XORed with P(span), P: bit j -> bit (7 j mod 11)
-        shift, blk = self.plan["layout"]
-        if not shift:
-            return i
-        span = np.asarray(i) >> shift
-        p = np.zeros_like(span)
-        for j in range(16):
-            p ^= ((span >> j) & 1) << ((7 * j) % 11)
-        return i ^ (p << blk)
+        return element_map(i, *self.plan["layout"])
 
     def run_pass(self, P, psi, lam=None, sink=None):
         # psi / lam are device-layout buffers; addresses are formed as the kernels form them:
