@@ -19,6 +19,7 @@ state (16 GiB) that stays resident in HBM.  Prints one JSON line:
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import math
 import os
@@ -77,12 +78,14 @@ def bench_forward(n, layers, steps, warmup):
     achieved = per_launch_bytes / (avg_ms * 1e6) if avg_ms > 0 else 0.0
     # sanity of the result
     norm = c.state.norm_sqr()
+    pieces = ctypes.c_int(0)
+    N.call("dmv_state_pieces", c.state.handle, ctypes.byref(pieces))
     log(f"[bench] forward {ms_step:.2f} ms/step, tile kernel {avg_ms:.3f} ms/launch, {achieved:.0f} GB/s, "
         f"|norm-1| {abs(norm - 1):.2e}")
     return {
         "gates": len(gates), "passes": passes, "phases": phases, "ops": ops, "hbm_bytes": hbm_bytes,
         "ms_step": ms_step, "launches": launches, "avg_launch_ms": avg_ms, "achieved": achieved,
-        "per_launch_bytes": per_launch_bytes, "norm_err": abs(norm - 1),
+        "per_launch_bytes": per_launch_bytes, "norm_err": abs(norm - 1), "state_pieces": pieces.value,
     }, c, gates
 
 
@@ -334,7 +337,7 @@ def main():
             "workload": f"{args.qubits}-qubit complex128 layered circuit, {args.layers} layers "
                         f"({fw['gates']} gates: H, RX, RY, RZ on every qubit, CNOT ladder), one fused forward per step",
             "qubits": args.qubits, "layers": args.layers, "gates": fw["gates"], "fused_passes": fw["passes"],
-            "state_bytes": 16 * 2 ** args.qubits, "parallelism": "single GPU",
+            "state_bytes": 16 * 2 ** args.qubits, "state_pieces": fw["state_pieces"], "parallelism": "single GPU",
         },
         "roofline": {
             "bound": "hbm", "achieved": round(fw["achieved"], 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",

@@ -8,6 +8,7 @@ int launch_tile_c128_other(int rlog, int tlog, bool nt, void *state, int n, Stat
 {
     if (tlog == 9 && rlog == 3) return run_tile<double, 3, 9>(nt, state, n, L, passes, phases, ops, pass_idx);
     if (tlog == 8 && rlog == 3) return run_tile<double, 3, 8>(nt, state, n, L, passes, phases, ops, pass_idx);
+    if (tlog == 7 && rlog == 4) return run_tile<double, 4, 7>(nt, state, n, L, passes, phases, ops, pass_idx);
     if (tlog == 8 && rlog == 2) return run_tile<double, 2, 8>(nt, state, n, L, passes, phases, ops, pass_idx);
     if (tlog == 6 && rlog == 5) return run_tile<double, 5, 6>(nt, state, n, L, passes, phases, ops, pass_idx);
     if (tlog == 6 && rlog == 4) return run_tile<double, 4, 6>(nt, state, n, L, passes, phases, ops, pass_idx);

@@ -103,7 +103,7 @@ def test_pipelined_tile_passes(native, n, grid, nt):
 TILE_CASES = [
     (20, 13, 9, 8, {}), (19, 12, 9, 4, {}), (20, 11, 8, 40, {}), (22, 12, 8, 2, {}), (20, 11, 6, 8, {}),
     (20, 10, 6, 16, {}), (21, 12, 8, 8, {"tile_xcd_log2": 0}), (20, 12, 8, 8, {"coalesce_qubits": 5}),
-    (20, 12, 8, 8, {"register_renaming": 0}),
+    (20, 12, 8, 8, {"register_renaming": 0}), (21, 11, 7, 8, {}), (21, 11, 7, 6, {"tile_xcd_log2": 0}),
 ]
 
 
