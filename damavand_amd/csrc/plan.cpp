@@ -709,7 +709,7 @@ int validate_plan(const Plan &plan, const PlanParams &pp)
                 }
                 const bool target_reg = register_bit(D.lay, pp, op.target) != kNoReg;
                 if (op_needs_register(op.kind)) {
-                    if (!target_reg || op.rmask == 0 || (op.rmask & ~regs) || op.tbit != kNoReg) {
+                    if (!target_reg || op.rmask == 0 || op.rmask >= (uint32_t)NR || op.tbit != kNoReg) {
                         set_error("internal: pair op without a register partner");
                         return DMV_ERR_STATE;
                     }

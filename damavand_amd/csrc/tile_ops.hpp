@@ -100,6 +100,12 @@ template <int M, class V, int NR> __device__ __forceinline__ void op_swap_m(V (&
 #define DMV_MASK_CASES(X)                                                                          \
     X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17)   \
     X(18) X(19) X(20) X(21) X(22) X(23) X(24) X(25) X(26) X(27) X(28) X(29) X(30) X(31)
+// The same without 1: the out-of-place switches send mask 1 and anything out
+// of range (which the host's plan validation excludes) to one case body, so
+// that every path writes its outputs in a case.
+#define DMV_MASK_CASES_FROM2(X)                                                                    \
+    X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17)        \
+    X(18) X(19) X(20) X(21) X(22) X(23) X(24) X(25) X(26) X(27) X(28) X(29) X(30) X(31)
 
 template <class V, class R, int NR>
 __device__ __forceinline__ void op_pair(V (&v)[NR], const DevOp *op)
@@ -194,20 +200,166 @@ __device__ __forceinline__ void op_gen_plain(V (&v)[NR], const DevOp *op)
     }
 }
 
+// ------------------------------------------------ out-of-place op bodies --
+// Each writes every element of `out` from `in`.  An op loop that alternates
+// two register arrays then merges the cases of an op's switch without
+// copies: with an in-place loop the compiler keeps the loop-carried array in
+// one register set and the cases' results in another, and moves all 2^RLOG
+// amplitudes between them twice per op (measured: 64 v_mov_b64 per op of the
+// complex128 k = 12 kernel, a third of its VALU instructions).
+
+template <class V, class R>
+__device__ __forceinline__ void apply2_to(V &oa, V &ob, const V &a, const V &b, const Mat<R> &M)
+{
+    oa.x = M.m00r * a.x - M.m00i * a.y + M.m01r * b.x - M.m01i * b.y;
+    oa.y = M.m00r * a.y + M.m00i * a.x + M.m01r * b.y + M.m01i * b.x;
+    ob.x = M.m10r * a.x - M.m10i * a.y + M.m11r * b.x - M.m11i * b.y;
+    ob.y = M.m10r * a.y + M.m10i * a.x + M.m11r * b.y + M.m11i * b.x;
+}
+
+template <class V, int NR> __device__ __forceinline__ void copy_regs(V (&out)[NR], const V (&in)[NR])
+{
+#pragma unroll
+    for (int r = 0; r < NR; r++)
+        out[r] = in[r];
+}
+
+template <int M, class V, class R, int NR>
+__device__ __forceinline__ void oop_gen_plain_m(V (&out)[NR], const V (&in)[NR], const Mat<R> &A)
+{
+    constexpr int HB = highest_bit(M);
+#pragma unroll
+    for (int p = 0; p < NR; p++) {
+        if (p & HB)
+            continue;
+        apply2_to(out[p], out[p ^ M], in[p], in[p ^ M], A);
+    }
+}
+
+template <int M, class V, class R, int NR>
+__device__ __forceinline__ void oop_gen_m(V (&out)[NR], const V (&in)[NR], const Mat<R> &A, uint32_t en, uint32_t fl)
+{
+    constexpr int HB = highest_bit(M);
+#pragma unroll
+    for (int p = 0; p < NR; p++) {
+        if (p & HB)
+            continue;
+        const bool f = (fl >> p) & 1u;
+        Mat<R> B;
+        B.m00r = f ? A.m11r : A.m00r; B.m00i = f ? A.m11i : A.m00i;
+        B.m01r = f ? A.m10r : A.m01r; B.m01i = f ? A.m10i : A.m01i;
+        B.m10r = f ? A.m01r : A.m10r; B.m10i = f ? A.m01i : A.m10i;
+        B.m11r = f ? A.m00r : A.m11r; B.m11i = f ? A.m00i : A.m11i;
+        V x, y;
+        apply2_to(x, y, in[p], in[p ^ M], B);
+        const bool ok = (en >> p) & 1u;
+        out[p] = ok ? x : in[p];
+        out[p ^ M] = ok ? y : in[p ^ M];
+    }
+}
+
+template <int M, class V, int NR>
+__device__ __forceinline__ void oop_swap_m(V (&out)[NR], const V (&in)[NR], uint32_t en)
+{
+    constexpr int HB = highest_bit(M);
+#pragma unroll
+    for (int p = 0; p < NR; p++) {
+        if (p & HB)
+            continue;
+        const bool ok = (en >> p) & 1u;
+        out[p] = ok ? in[p ^ M] : in[p];
+        out[p ^ M] = ok ? in[p] : in[p ^ M];
+    }
+}
+
+template <class V, class R, int NR>
+__device__ __forceinline__ void oop_diag(V (&out)[NR], const V (&in)[NR], const DevOp *op, uint32_t tid)
+{
+    const R d0r = (R)op->m[0], d0i = (R)op->m[1], d1r = (R)op->m[6], d1i = (R)op->m[7];
+    const uint32_t en = op->en;
+    const uint32_t all = NR >= 32 ? 0xFFFFFFFFu : ((1u << NR) - 1);
+    if (op->tbit == kNoReg) {
+        const uint32_t ones = op->rmask;
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            const bool one = (ones >> r) & 1u;
+            const V x = cmul(in[r], one ? d1r : d0r, one ? d1i : d0i);
+            out[r] = (en == all || ((en >> r) & 1u)) ? x : in[r];
+        }
+    } else {
+        const bool one = (tid >> op->tbit) & 1u;
+        const R dr = one ? d1r : d0r, di = one ? d1i : d0i;
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            const V x = cmul(in[r], dr, di);
+            out[r] = (en == all || ((en >> r) & 1u)) ? x : in[r];
+        }
+    }
+}
+
+// One op from `in` into `out`; lanes whose thread-bit controls do not hold
+// keep their amplitudes.
+template <class V, class R, int NR>
+__device__ __forceinline__ void oop_op(V (&out)[NR], const V (&in)[NR], const DevOp *op, uint32_t tid)
+{
+    const uint32_t en = op->en, fl = op->fl;
+    const Mat<R> A = load_mat<R>(op->m);
+    // masks at or above NR have empty cases; the clamp lets the compiler drop them
+    const uint32_t m = op->rmask < (uint32_t)NR ? op->rmask : 1u;
+    if (op->plain) {
+        switch (m) {
+#define DMV_OOP_PLAIN_CASE(M)                                                                      \
+    case M:                                                                                        \
+        if constexpr (M < NR)                                                                      \
+            oop_gen_plain_m<M>(out, in, A);                                                        \
+        break;
+            DMV_MASK_CASES_FROM2(DMV_OOP_PLAIN_CASE)
+#undef DMV_OOP_PLAIN_CASE
+        default: oop_gen_plain_m<1>(out, in, A); break;
+        }
+    } else if (op->kind == OP_DIAG) {
+        oop_diag<V, R>(out, in, op, tid);
+    } else if (op->kind == OP_GEN) {
+        switch (m) {
+#define DMV_OOP_GEN_CASE(M)                                                                        \
+    case M:                                                                                        \
+        if constexpr (M < NR)                                                                      \
+            oop_gen_m<M>(out, in, A, en, fl);                                                      \
+        break;
+            DMV_MASK_CASES_FROM2(DMV_OOP_GEN_CASE)
+#undef DMV_OOP_GEN_CASE
+        default: oop_gen_m<1>(out, in, A, en, fl); break;
+        }
+    } else {
+        switch (m) {
+#define DMV_OOP_SWAP_CASE(M)                                                                       \
+    case M:                                                                                        \
+        if constexpr (M < NR)                                                                      \
+            oop_swap_m<M>(out, in, en);                                                            \
+        break;
+            DMV_MASK_CASES_FROM2(DMV_OOP_SWAP_CASE)
+#undef DMV_OOP_SWAP_CASE
+        default: oop_swap_m<1>(out, in, en); break;
+        }
+    }
+    if (op->ctrl_thr && !thread_controls_hold(op, tid))
+        copy_regs(out, in);
+}
+
+// The ops [ob, oe) of a phase, alternating between v and a second array.
 template <class V, class R, int RLOG>
 __device__ __forceinline__ void run_ops(V (&v)[1 << RLOG], const DevOp *__restrict__ ops, uint32_t ob, uint32_t oe,
                                         uint32_t tid)
 {
-    for (uint32_t o = ob; o < oe; o++) {
-        const DevOp *op = ops + o;
-        if (!thread_controls_hold(op, tid))
-            continue;
-        if (op->plain)
-            op_gen_plain<V, R>(v, op);
-        else if (op->kind == OP_DIAG)
-            op_diag<V, R>(v, op, tid);
-        else
-            op_pair<V, R>(v, op);
+    V w[1 << RLOG];
+    uint32_t o = ob;
+    for (; o + 1 < oe; o += 2) {
+        oop_op<V, R>(w, v, ops + o, tid);
+        oop_op<V, R>(v, w, ops + o + 1, tid);
+    }
+    if (o < oe) {
+        oop_op<V, R>(w, v, ops + o, tid);
+        copy_regs(v, w);
     }
 }
 
