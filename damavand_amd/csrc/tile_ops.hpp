@@ -237,7 +237,8 @@ __device__ __forceinline__ void oop_gen_plain_m(V (&out)[NR], const V (&in)[NR],
 }
 
 template <int M, class V, class R, int NR>
-__device__ __forceinline__ void oop_gen_m(V (&out)[NR], const V (&in)[NR], const Mat<R> &A, uint32_t en, uint32_t fl)
+__device__ __forceinline__ void oop_gen_m(V (&out)[NR], const V (&in)[NR], const Mat<R> &A, uint32_t en, uint32_t fl,
+                                          bool hold)
 {
     constexpr int HB = highest_bit(M);
 #pragma unroll
@@ -252,28 +253,28 @@ __device__ __forceinline__ void oop_gen_m(V (&out)[NR], const V (&in)[NR], const
         B.m11r = f ? A.m00r : A.m11r; B.m11i = f ? A.m00i : A.m11i;
         V x, y;
         apply2_to(x, y, in[p], in[p ^ M], B);
-        const bool ok = (en >> p) & 1u;
+        const bool ok = ((en >> p) & 1u) && hold;
         out[p] = ok ? x : in[p];
         out[p ^ M] = ok ? y : in[p ^ M];
     }
 }
 
 template <int M, class V, int NR>
-__device__ __forceinline__ void oop_swap_m(V (&out)[NR], const V (&in)[NR], uint32_t en)
+__device__ __forceinline__ void oop_swap_m(V (&out)[NR], const V (&in)[NR], uint32_t en, bool hold)
 {
     constexpr int HB = highest_bit(M);
 #pragma unroll
     for (int p = 0; p < NR; p++) {
         if (p & HB)
             continue;
-        const bool ok = (en >> p) & 1u;
+        const bool ok = ((en >> p) & 1u) && hold;
         out[p] = ok ? in[p ^ M] : in[p];
         out[p ^ M] = ok ? in[p] : in[p ^ M];
     }
 }
 
 template <class V, class R, int NR>
-__device__ __forceinline__ void oop_diag(V (&out)[NR], const V (&in)[NR], const DevOp *op, uint32_t tid)
+__device__ __forceinline__ void oop_diag(V (&out)[NR], const V (&in)[NR], const DevOp *op, uint32_t tid, bool hold)
 {
     const R d0r = (R)op->m[0], d0i = (R)op->m[1], d1r = (R)op->m[6], d1i = (R)op->m[7];
     const uint32_t en = op->en;
@@ -284,7 +285,7 @@ __device__ __forceinline__ void oop_diag(V (&out)[NR], const V (&in)[NR], const 
         for (int r = 0; r < NR; r++) {
             const bool one = (ones >> r) & 1u;
             const V x = cmul(in[r], one ? d1r : d0r, one ? d1i : d0i);
-            out[r] = (en == all || ((en >> r) & 1u)) ? x : in[r];
+            out[r] = ((en == all || ((en >> r) & 1u)) && hold) ? x : in[r];
         }
     } else {
         const bool one = (tid >> op->tbit) & 1u;
@@ -292,17 +293,19 @@ __device__ __forceinline__ void oop_diag(V (&out)[NR], const V (&in)[NR], const 
 #pragma unroll
         for (int r = 0; r < NR; r++) {
             const V x = cmul(in[r], dr, di);
-            out[r] = (en == all || ((en >> r) & 1u)) ? x : in[r];
+            out[r] = ((en == all || ((en >> r) & 1u)) && hold) ? x : in[r];
         }
     }
 }
 
 // One op from `in` into `out`; lanes whose thread-bit controls do not hold
-// keep their amplitudes.
+// keep their amplitudes: through the selects the op makes anyway, or by a
+// copy after a plain GEN, which makes none.
 template <class V, class R, int NR>
 __device__ __forceinline__ void oop_op(V (&out)[NR], const V (&in)[NR], const DevOp *op, uint32_t tid)
 {
     const uint32_t en = op->en, fl = op->fl;
+    const bool hold = thread_controls_hold(op, tid);
     const Mat<R> A = load_mat<R>(op->m);
     // masks at or above NR have empty cases; the clamp lets the compiler drop them
     const uint32_t m = op->rmask < (uint32_t)NR ? op->rmask : 1u;
@@ -317,33 +320,33 @@ __device__ __forceinline__ void oop_op(V (&out)[NR], const V (&in)[NR], const De
 #undef DMV_OOP_PLAIN_CASE
         default: oop_gen_plain_m<1>(out, in, A); break;
         }
+        if (op->ctrl_thr && !hold)
+            copy_regs(out, in);
     } else if (op->kind == OP_DIAG) {
-        oop_diag<V, R>(out, in, op, tid);
+        oop_diag<V, R>(out, in, op, tid, hold);
     } else if (op->kind == OP_GEN) {
         switch (m) {
 #define DMV_OOP_GEN_CASE(M)                                                                        \
     case M:                                                                                        \
         if constexpr (M < NR)                                                                      \
-            oop_gen_m<M>(out, in, A, en, fl);                                                      \
+            oop_gen_m<M>(out, in, A, en, fl, hold);                                                \
         break;
             DMV_MASK_CASES_FROM2(DMV_OOP_GEN_CASE)
 #undef DMV_OOP_GEN_CASE
-        default: oop_gen_m<1>(out, in, A, en, fl); break;
+        default: oop_gen_m<1>(out, in, A, en, fl, hold); break;
         }
     } else {
         switch (m) {
 #define DMV_OOP_SWAP_CASE(M)                                                                       \
     case M:                                                                                        \
         if constexpr (M < NR)                                                                      \
-            oop_swap_m<M>(out, in, en);                                                            \
+            oop_swap_m<M>(out, in, en, hold);                                                      \
         break;
             DMV_MASK_CASES_FROM2(DMV_OOP_SWAP_CASE)
 #undef DMV_OOP_SWAP_CASE
-        default: oop_swap_m<1>(out, in, en); break;
+        default: oop_swap_m<1>(out, in, en, hold); break;
         }
     }
-    if (op->ctrl_thr && !thread_controls_hold(op, tid))
-        copy_regs(out, in);
 }
 
 // The ops [ob, oe) of a phase, alternating between v and a second array.
