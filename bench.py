@@ -300,7 +300,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the state lives on a single MI355X and this build has no "
-              "multi-GPU path (see DESIGN.md for how it would shard). Run with --gpus 1.", file=sys.stderr)
+              "multi-GPU path. Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     from damavand_amd import _native as N
