@@ -1,10 +1,12 @@
 """Builds libdamavand-gpu.so for gfx950 in-tree (damavand_amd/libdamavand-gpu.so).
 
-    python -m damavand_amd.build [--jobs N] [--verbose]
+    python -m damavand_amd.build [--jobs N] [--verbose] [--incremental]
 
 Every source under damavand_amd/csrc is compiled with hipcc (objects under
 damavand_amd/_build/) and linked into one shared library exporting the C ABI
-declared in include/damavand_gpu.h.  No GPU is needed to build.
+declared in include/damavand_gpu.h.  No GPU is needed to build.  By default
+every unit is compiled; --incremental recompiles only units whose object is
+older than the unit or than any header under csrc/ and include/.
 """
 from __future__ import annotations
 
@@ -67,15 +69,39 @@ def _compile(src: str, verbose: bool) -> str:
     return obj
 
 
-def build(jobs: int = 0, verbose: bool = False) -> str:
+def _deps(path: str, seen=None) -> set:
+    """The file and every quoted #include it reaches (csrc/ and include/)."""
+    seen = set() if seen is None else seen
+    if path in seen or not os.path.exists(path):
+        return seen
+    seen.add(path)
+    with open(path) as f:
+        for line in f:
+            line = line.strip()
+            if line.startswith("#include") and '"' in line:
+                name = line.split('"')[1]
+                _deps(os.path.normpath(os.path.join(os.path.dirname(path), name)), seen)
+    return seen
+
+
+def _up_to_date(src: str) -> bool:
+    obj = os.path.join(OBJ, os.path.basename(src) + ".o")
+    if not os.path.exists(obj):
+        return False
+    t = os.path.getmtime(obj)
+    return all(t >= os.path.getmtime(d) for d in _deps(src) | {__file__})
+
+
+def build(jobs: int = 0, verbose: bool = False, incremental: bool = False) -> str:
     os.makedirs(OBJ, exist_ok=True)
     srcs = sources()
     jobs = jobs or min(8, os.cpu_count() or 1)
+    todo = [s for s in srcs if not (incremental and _up_to_date(s))]
     # longest compiles first (the kernel instantiation units), so they overlap
-    order = sorted(srcs, key=lambda s: -os.path.getsize(s) if s.endswith(".hip") else 0)
+    order = sorted(todo, key=lambda s: -os.path.getsize(s) if s.endswith(".hip") else 0)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        done = dict(zip(order, ex.map(lambda s: _compile(s, verbose), order)))
-    objs = [done[s] for s in srcs]
+        list(ex.map(lambda s: _compile(s, verbose), order))
+    objs = [os.path.join(OBJ, os.path.basename(s) + ".o") for s in srcs]
     tmp = LIB + ".tmp"
     cmd = [hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs
     if verbose:
@@ -91,8 +117,9 @@ def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=0)
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--incremental", action="store_true")
     a = ap.parse_args()
-    print(build(a.jobs, a.verbose))
+    print(build(a.jobs, a.verbose, a.incremental))
 
 
 if __name__ == "__main__":

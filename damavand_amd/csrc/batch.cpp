@@ -29,7 +29,15 @@ int batch_forward_adjoint(int count, int dtype, const int *n, const int *gate_of
     DMV_CHECK_ARG(n && gate_offset && gate_kind && gate_control && gate_target && gate_param && coef_offset && coef &&
                       param_offset && energy && grad,
                   "null batch array");
-    DMV_TRY(ensure_device());
+    // the offsets index host arrays here and device buffers in the kernel:
+    // gate ranges must be ordered, coefficient ranges non-negative, and the
+    // gradient offsets the exclusive prefix sum of the parameter counts
+    // (checked once the counts are known, below)
+    DMV_CHECK_ARG(gate_offset[0] >= 0, "gate_offset[0] is negative");
+    for (int i = 0; i < count; i++) {
+        DMV_CHECK_ARG(gate_offset[i + 1] >= gate_offset[i], "gate_offset decreases at circuit %d", i);
+        DMV_CHECK_ARG(coef_offset[i] >= 0, "coef_offset[%d] is negative", i);
+    }
     int rlog, tlog;
     batch_tile_shape(dtype, &rlog, &tlog);
     const int K = rlog + tlog;
@@ -57,7 +65,6 @@ int batch_forward_adjoint(int count, int dtype, const int *n, const int *gate_of
     auto build_one = [&](int i, Built &B) -> int {
         const int ni = n[i];
         DMV_CHECK_ARG(ni >= 2 && ni <= 24, "batch circuit %d: qubits must be in [2, 24], got %d", i, ni);
-        DMV_CHECK_ARG(coef_offset[i] >= 0, "bad coefficient offset");
         std::vector<Gate> gates;
         for (int g = gate_offset[i]; g < gate_offset[i + 1]; g++) {
             Gate G;
@@ -128,6 +135,11 @@ int batch_forward_adjoint(int count, int dtype, const int *n, const int *gate_of
         }
         const int ni = n[i];
         nmax = std::max(nmax, ni);
+        if (param_offset[i] != total_params) {
+            set_error("param_offset[%d] is %d; it must be the number of parameters of the circuits before it (%lld)", i,
+                      param_offset[i], (long long)total_params);
+            return DMV_ERR_ARG;
+        }
         total_params += B.np;
         BatchCircuit &C = circ[i];
         C.n = (uint32_t)ni;
@@ -176,6 +188,8 @@ int batch_forward_adjoint(int count, int dtype, const int *n, const int *gate_of
         order[i] = (uint32_t)i;
     std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return cost[x] > cost[y]; });
 
+    // host work and argument checks are done: only now touch the device
+    DMV_TRY(ensure_device());
     const int per_cu = (int)tune_value("batch_wgs_per_cu", 2);
     uint32_t grid = (uint32_t)std::min<int64_t>(count, (int64_t)device_cus() * std::max(1, per_cu));
     const uint64_t slot_amps = uint64_t(1) << std::max(nmax, K);

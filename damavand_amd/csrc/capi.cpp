@@ -312,11 +312,9 @@ int dmv_expect_zz(dmv_state *s, const double *a, int n_pairs, const int *pi, con
     CHECK_STATE(s);
     DMV_CHECK_ARG(out != nullptr, "null output");
     DMV_TRY(ensure_device());
-    static double *terms = nullptr;
-    static uint64_t cap = 0;
     const double *da, *db;
     const int *dpi, *dpj;
-    DMV_TRY(upload_terms(&terms, &cap, s->n, a, n_pairs, pi, pj, b, &da, &db, &dpi, &dpj));
+    DMV_TRY(upload_terms(&s->terms, &s->cap_terms, s->n, a, n_pairs, pi, pj, b, &da, &db, &dpi, &dpj));
     prof_begin(DMV_KC_REDUCE);
     DMV_TRY(launch_expect_zz(s->dtype, s->amp, s->n, s->layout, da, n_pairs, dpi, dpj, db, s->ws, s->scalars));
     prof_end(DMV_KC_REDUCE, (double)s->bytes);
@@ -479,13 +477,21 @@ int dmv_circuit_set_parameters(dmv_circuit *c, const double *p, int count)
 {
     CHECK_CIRCUIT(c);
     DMV_CHECK_ARG(count >= 0 && (p != nullptr || count == 0), "bad parameter array");
+    // every value is checked before any gate changes, so a rejected call
+    // leaves the circuit (and its cached plans) as they were
+    int np = 0;
+    for (const Gate &g : c->gates)
+        if (gate_is_parametric(g.kind))
+            np++;
+    const int used = count < np ? count : np;
+    for (int k = 0; k < used; k++)
+        DMV_CHECK_ARG(std::isfinite(p[k]), "non-finite parameter %d", k);
     int k = 0;
     for (Gate &g : c->gates) {
         if (!gate_is_parametric(g.kind))
             continue;
-        if (k >= count)
+        if (k >= used)
             break;
-        DMV_CHECK_ARG(std::isfinite(p[k]), "non-finite parameter %d", k);
         g.p[0] = p[k++];
     }
     c->version++;

@@ -202,40 +202,96 @@ int launch_sum(const double *in, uint64_t count, double *out)
     return check_launch("sum");
 }
 
-// <Z_q>: per block of 2^B amplitudes, the total and, for q < B, the mass
-// with bit q set.  Bits >= B are constant over a block.
+// <Z_q>: each workgroup reads 2^G consecutive blocks of 4096 amplitudes
+// (G = group bits) and writes the total mass of its 2^(12+G) amplitudes and,
+// for every q < 12 + G, the mass with bit q set; bits >= 12 + G are constant
+// over a workgroup and are added by k_expect_z_final.  Thread t reads
+// amplitudes t + 256 k (k < 16) of every block, so bits 0-7 of an index are
+// the thread's (bits 0-5 the lane, 6-7 the wave) and bits 8-11 are k's: a
+// thread keeps only its total, the four k-bit masses and the G group-bit
+// masses, and the lane-bit masses come from the wave totals.  One barrier
+// per workgroup.
+constexpr int kZGroupMax = 4;
+constexpr int kZStrideMax = 1 + kBlockBits + kZGroupMax;  // doubles per workgroup partial
+static_assert(RT == 256 && RT * 16 == (1 << kBlockBits), "expect_z thread mapping");
+
 template <class V>
-__global__ void __launch_bounds__(RT) k_expect_z_part(const V *__restrict__ s, uint32_t B, double *partials,
-                                                     StateLayout L)
+__global__ void __launch_bounds__(RT) k_expect_z_part(const V *__restrict__ s, uint32_t B, uint32_t G,
+                                                     double *partials, StateLayout L)
 {
-    __shared__ double red[RT / 64];
-    const uint64_t eb = elem((uint64_t)blockIdx.x << B, L);
+    constexpr int NW = RT / 64;
+    constexpr int NV = 1 + 6 + 4 + kZGroupMax;  // total, lane bits, k bits, group bits
+    __shared__ double red[NW][NV];
+    const uint32_t bw = B + G;
+    const uint64_t eb = elem((uint64_t)blockIdx.x << bw, L);  // the group lies in one span
     const uint32_t span = 1u << B;
-    double tot = 0.0, m[kBlockBits];
+    double tot = 0.0, hk[4] = {0.0, 0.0, 0.0, 0.0}, hg[kZGroupMax] = {0.0, 0.0, 0.0, 0.0};
+    for (uint32_t g = 0; g < (1u << G); g++) {
+        double p[16];
 #pragma unroll
-    for (int q = 0; q < kBlockBits; q++)
-        m[q] = 0.0;
-    for (uint32_t i = threadIdx.x; i < span; i += RT) {
-        const double p = norm2(s[eb ^ i]);
-        tot += p;
+        for (int k = 0; k < 16; k++) {
+            const uint32_t i = threadIdx.x + (uint32_t)k * RT;
+            p[k] = i < span ? norm2(s[eb ^ ((uint64_t)g << B) ^ i]) : 0.0;
+        }
+        double bt = 0.0;
 #pragma unroll
-        for (int q = 0; q < kBlockBits; q++)
-            if ((i >> q) & 1u)
-                m[q] += p;
+        for (int k = 0; k < 16; k++) {
+            bt += p[k];
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                if ((k >> j) & 1)
+                    hk[j] += p[k];
+        }
+        tot += bt;
+#pragma unroll
+        for (int j = 0; j < kZGroupMax; j++)
+            if ((g >> j) & 1u)
+                hg[j] += bt;
     }
-    double *out = partials + (uint64_t)blockIdx.x * (1 + kBlockBits);
-    const double t = block_sum<RT>(tot, red);
-    if (threadIdx.x == 0)
-        out[0] = t;
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    double v[NV];
+    v[0] = wave_sum(tot);
 #pragma unroll
-    for (int q = 0; q < kBlockBits; q++) {
-        const double v = block_sum<RT>(m[q], red);
-        if (threadIdx.x == 0)
-            out[1 + q] = v;
+    for (int q = 0; q < 6; q++)
+        v[1 + q] = wave_sum(((lane >> q) & 1u) ? tot : 0.0);
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+        v[7 + j] = wave_sum(hk[j]);
+#pragma unroll
+    for (int j = 0; j < kZGroupMax; j++)
+        v[11 + j] = wave_sum(hg[j]);
+    if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; i++)
+            red[w][i] = v[i];
+    }
+    __syncthreads();
+    // thread 0: total; thread 1 + q: mass with bit q set (q < B + G)
+    const int q = (int)threadIdx.x - 1;
+    if (q < (int)bw) {
+        double acc = 0.0;
+#pragma unroll
+        for (int ww = 0; ww < NW; ww++) {
+            double x;
+            if (q < 0)
+                x = red[ww][0];
+            else if (q < 6)
+                x = red[ww][1 + q];
+            else if (q < 8)
+                x = ((ww >> (q - 6)) & 1) ? red[ww][0] : 0.0;
+            else if (q < kBlockBits)
+                x = red[ww][7 + q - 8];
+            else
+                x = red[ww][11 + q - kBlockBits];
+            acc += x;
+        }
+        partials[(uint64_t)blockIdx.x * (1 + bw) + threadIdx.x] = acc;
     }
 }
 
-__global__ void __launch_bounds__(RT) k_expect_z_final(const double *__restrict__ partials, uint64_t nblk, uint32_t B,
+// Sums the workgroup partials (stride 1 + bw, bw = bits inside a workgroup)
+// into <Z_q> = total - 2 mass(bit q set), one workgroup, fixed order.
+__global__ void __launch_bounds__(RT) k_expect_z_final(const double *__restrict__ partials, uint64_t ngroups, uint32_t bw,
                                                        uint32_t n, double *out)
 {
     __shared__ double red[RT / 64];
@@ -243,39 +299,54 @@ __global__ void __launch_bounds__(RT) k_expect_z_final(const double *__restrict_
 #pragma unroll
     for (int q = 0; q < kMaxQubits; q++)
         m[q] = 0.0;
-    for (uint64_t b = threadIdx.x; b < nblk; b += RT) {
-        const double *pb = partials + b * (1 + kBlockBits);
+    for (uint64_t b = threadIdx.x; b < ngroups; b += RT) {
+        const double *pb = partials + b * (1 + bw);
         const double t = pb[0];
         tot += t;
 #pragma unroll
         for (int q = 0; q < kMaxQubits; q++) {
-            if ((uint32_t)q < B)
-                m[q] += pb[1 + (q < kBlockBits ? q : 0)];
-            else if ((uint32_t)q < n && ((b >> (q - B)) & 1u))
+            if ((uint32_t)q < bw)
+                m[q] += pb[1 + (q < kZStrideMax - 1 ? q : 0)];
+            else if ((uint32_t)q < n && ((b >> (q - bw)) & 1u))
                 m[q] += t;
         }
     }
     const double T = block_sum<RT>(tot, red);
 #pragma unroll
     for (int q = 0; q < kMaxQubits; q++) {
+        if ((uint32_t)q >= n)
+            break;
         const double v = block_sum<RT>(m[q], red);
-        if (threadIdx.x == 0 && (uint32_t)q < n)
+        if (threadIdx.x == 0)
             out[q] = T - 2.0 * v;
     }
+}
+
+// Group bits of <Z_q>: up to four blocks per workgroup while the grid keeps
+// at least 2^11 workgroups, within one span of the layout.
+static uint32_t expect_z_group_bits(int n, int B, StateLayout L)
+{
+    int g = n - B - 11;
+    g = g < 0 ? 0 : (g > kZGroupMax ? kZGroupMax : g);
+    if (L.shift && B + g > (int)L.shift)
+        g = (int)L.shift - B;
+    return (uint32_t)g;
 }
 
 int launch_expect_z(int dtype, const void *state, int n, StateLayout L, double *partials, double *out)
 {
     const int B = n < kBlockBits ? n : kBlockBits;
-    const uint64_t nblk = uint64_t(1) << (n - B);
+    const uint32_t G = expect_z_group_bits(n, B, L);
+    const uint64_t ngroups = uint64_t(1) << (n - B - (int)G);
     if (dtype == DMV_C64)
-        hipLaunchKernelGGL(k_expect_z_part<c64>, dim3((uint32_t)nblk), dim3(RT), 0, stream(), (const c64 *)state,
-                           (uint32_t)B, partials, L);
+        hipLaunchKernelGGL(k_expect_z_part<c64>, dim3((uint32_t)ngroups), dim3(RT), 0, stream(), (const c64 *)state,
+                           (uint32_t)B, G, partials, L);
     else
-        hipLaunchKernelGGL(k_expect_z_part<c128>, dim3((uint32_t)nblk), dim3(RT), 0, stream(), (const c128 *)state,
-                           (uint32_t)B, partials, L);
+        hipLaunchKernelGGL(k_expect_z_part<c128>, dim3((uint32_t)ngroups), dim3(RT), 0, stream(), (const c128 *)state,
+                           (uint32_t)B, G, partials, L);
     DMV_TRY(check_launch("<Z> partials"));
-    hipLaunchKernelGGL(k_expect_z_final, dim3(1), dim3(RT), 0, stream(), partials, nblk, (uint32_t)B, (uint32_t)n, out);
+    hipLaunchKernelGGL(k_expect_z_final, dim3(1), dim3(RT), 0, stream(), partials, ngroups, (uint32_t)B + G, (uint32_t)n,
+                       out);
     return check_launch("<Z> final");
 }
 
@@ -297,12 +368,13 @@ __global__ void __launch_bounds__(RT) k_obs(const V *__restrict__ s, V *__restri
     __shared__ double red[RT / 64];
     __shared__ double sa[kMaxQubits];
     __shared__ double sb[kMaxPairs];
-    __shared__ uint64_t smask[kMaxPairs];
+    __shared__ uint8_t spi[kMaxPairs], spj[kMaxPairs];
     for (uint32_t q = threadIdx.x; q < n; q += RT)
         sa[q] = T.a ? T.a[q] : 0.0;
     for (int t = threadIdx.x; t < T.n_pairs; t += RT) {
         sb[t] = T.b[t];
-        smask[t] = (uint64_t(1) << T.pi[t]) | (uint64_t(1) << T.pj[t]);
+        spi[t] = (uint8_t)T.pi[t];
+        spj[t] = (uint8_t)T.pj[t];
     }
     __syncthreads();
     const uint64_t base = (uint64_t)blockIdx.x << B;
@@ -315,7 +387,7 @@ __global__ void __launch_bounds__(RT) k_obs(const V *__restrict__ s, V *__restri
         for (uint32_t q = 0; q < n; q++)
             h += ((x >> q) & 1u) ? -sa[q] : sa[q];
         for (int t = 0; t < T.n_pairs; t++)
-            h += (__popcll(x & smask[t]) & 1) ? -sb[t] : sb[t];
+            h += (((x >> spi[t]) ^ (x >> spj[t])) & 1u) ? -sb[t] : sb[t];
         const V v = s[e0 ^ i];
         acc += h * norm2(v);
         if constexpr (WRITE) {

@@ -1,4 +1,4 @@
-// Fused tile passes, complex128, the other register/thread shapes.
+// Fused tile passes, complex128, the other 256/512/128-thread shapes.
 #include "k_tile_impl.hpp"
 
 namespace dmv {
@@ -10,10 +10,7 @@ int launch_tile_c128_other(int rlog, int tlog, bool nt, void *state, int n, Stat
     if (tlog == 8 && rlog == 3) return run_tile<double, 3, 8>(nt, state, n, L, passes, phases, ops, pass_idx);
     if (tlog == 7 && rlog == 4) return run_tile<double, 4, 7>(nt, state, n, L, passes, phases, ops, pass_idx);
     if (tlog == 8 && rlog == 2) return run_tile<double, 2, 8>(nt, state, n, L, passes, phases, ops, pass_idx);
-    if (tlog == 6 && rlog == 5) return run_tile<double, 5, 6>(nt, state, n, L, passes, phases, ops, pass_idx);
-    if (tlog == 6 && rlog == 4) return run_tile<double, 4, 6>(nt, state, n, L, passes, phases, ops, pass_idx);
-    if (tlog == 6 && rlog == 3) return run_tile<double, 3, 6>(nt, state, n, L, passes, phases, ops, pass_idx);
-    return kNotHandled;
+    return launch_tile_c128_wave(rlog, tlog, nt, state, n, L, passes, phases, ops, pass_idx);
 }
 
 }  // namespace dmv

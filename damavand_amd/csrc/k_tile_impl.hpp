@@ -23,19 +23,6 @@ k_tile(typename Cplx<R>::T *__restrict__ state, const DevPass *__restrict__ pass
     tile_body<R, RLOG, TLOG, NT>(state, passes + pass_idx, phases, ops, tile, lds, L);
 }
 
-// Persistent form: gridDim.x workgroups stride over the tiles with the next
-// tile's loads in flight during the current tile's compute.
-template <class R, int RLOG, int TLOG, bool NT>
-__global__ void __launch_bounds__(1 << TLOG)
-k_tile_pipe(typename Cplx<R>::T *__restrict__ state, const DevPass *__restrict__ passes,
-            const DevPhase *__restrict__ phases, const DevOp *__restrict__ ops, uint32_t pass_idx, uint64_t n_tiles,
-            StateLayout L)
-{
-    using V = typename Cplx<R>::T;
-    __shared__ V lds[1 << (RLOG + TLOG)];
-    tile_stream<R, RLOG, TLOG, NT>(state, passes + pass_idx, phases, ops, blockIdx.x, gridDim.x, n_tiles, lds, L);
-}
-
 // ------------------------------------------------------------- launchers --
 template <class R, int RLOG, int TLOG>
 static int run_tile(bool nt, void *state, int n, StateLayout L, const DevPass *passes, const DevPhase *phases, const DevOp *ops,
@@ -43,19 +30,6 @@ static int run_tile(bool nt, void *state, int n, StateLayout L, const DevPass *p
 {
     using V = typename Cplx<R>::T;
     const uint64_t tiles = uint64_t(1) << (n - (RLOG + TLOG));
-    const int64_t per_cu = tune_value("tile_wgs_per_cu", 2);
-    const int64_t forced = tune_value("tile_grid", 0);  // explicit persistent grid (tests)
-    const uint64_t grid64 = forced > 0 ? (uint64_t)forced : (uint64_t)device_cus() * (uint64_t)(per_cu > 0 ? per_cu : 1);
-    if (tune_value("tile_pipeline", 0) && per_cu > 0 && tiles >= 2 * grid64) {
-        const uint32_t grid = (uint32_t)grid64;
-        if (nt)
-            hipLaunchKernelGGL((k_tile_pipe<R, RLOG, TLOG, true>), dim3(grid), dim3(1 << TLOG), 0, stream(),
-                               (V *)state, passes, phases, ops, pass_idx, tiles, L);
-        else
-            hipLaunchKernelGGL((k_tile_pipe<R, RLOG, TLOG, false>), dim3(grid), dim3(1 << TLOG), 0, stream(),
-                               (V *)state, passes, phases, ops, pass_idx, tiles, L);
-        return check_launch("pipelined tile pass");
-    }
     const int64_t xl = tune_value("tile_xcd_log2", 3);  // 8 XCDs per MI355X
     const uint32_t xcd_log2 = (xl > 0 && xl <= 4 && tiles >= (uint64_t(1) << xl)) ? (uint32_t)xl : 0u;
     if (nt)
@@ -74,6 +48,8 @@ int launch_tile_c128_main(int rlog, int tlog, bool nt, void *state, int n, State
                           const DevPhase *phases, const DevOp *ops, uint32_t pass_idx);
 int launch_tile_c128_other(int rlog, int tlog, bool nt, void *state, int n, StateLayout L, const DevPass *passes,
                            const DevPhase *phases, const DevOp *ops, uint32_t pass_idx);
+int launch_tile_c128_wave(int rlog, int tlog, bool nt, void *state, int n, StateLayout L, const DevPass *passes,
+                          const DevPhase *phases, const DevOp *ops, uint32_t pass_idx);
 int launch_tile_c64(int rlog, int tlog, bool nt, void *state, int n, StateLayout L, const DevPass *passes,
                     const DevPhase *phases, const DevOp *ops, uint32_t pass_idx);
 

@@ -1,6 +1,7 @@
 // Device context, profiling, memory, states and circuit execution.
 #include "runtime.hpp"
 
+#include <atomic>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -36,21 +37,33 @@ struct ProfRec {
     double bytes;
 };
 
+// Process-wide library state.  Calls on different handles may come from
+// different threads: initialisation is double-checked on an atomic, and the
+// profiling records and the tuning table each have their own lock.  All work
+// goes to one library stream (a thread capturing a hipGraph launches on its
+// own capture stream instead, see stream()).
 struct Context {
     std::mutex mu;
-    bool init = false;
+    std::atomic<bool> init{false};
     int device = 0;
     hipStream_t stream = nullptr;
-    bool prof = false;
+    std::atomic<bool> prof{false};
+    std::mutex prof_mu;  // pending, free_events and the totals
     std::vector<ProfRec> pending;
     std::vector<hipEvent_t> free_events;
-    hipEvent_t open = nullptr;
     int64_t launches[DMV_KC_COUNT] = {};
     double ms[DMV_KC_COUNT] = {};
     double bytes[DMV_KC_COUNT] = {};
+    std::mutex tune_mu;  // tune
     std::map<std::string, int64_t> tune;
-    uint64_t tune_epoch = 0;  // bumped by every tune_set: cached plans rebuild
+    std::atomic<uint64_t> tune_epoch{0};  // bumped by every tune_set: cached plans rebuild
 };
+
+// Per thread: the event a prof_begin opened, whether profiling is held off
+// (inside a graph capture), and the stream a capture launches on.
+static thread_local hipEvent_t t_open = nullptr;
+static thread_local bool t_prof_off = false;
+static thread_local hipStream_t t_capture = nullptr;
 
 static Context &ctx()
 {
@@ -61,9 +74,9 @@ static Context &ctx()
 int ensure_device()
 {
     Context &C = ctx();
-    if (!C.init) {
+    if (!C.init.load(std::memory_order_acquire)) {
         std::lock_guard<std::mutex> lk(C.mu);
-        if (!C.init) {
+        if (!C.init.load(std::memory_order_relaxed)) {
             int count = 0;
             hipError_t e = hipGetDeviceCount(&count);
             if (e != hipSuccess || count <= 0) {
@@ -80,7 +93,7 @@ int ensure_device()
                 set_error("hipStreamCreate: %s", hipGetErrorString(e));
                 return DMV_ERR_HIP;
             }
-            C.init = true;
+            C.init.store(true, std::memory_order_release);
         }
     }
     hipError_t e = hipSetDevice(C.device);
@@ -91,19 +104,21 @@ int ensure_device()
     return DMV_OK;
 }
 
-hipStream_t stream() { return ctx().stream; }
+hipStream_t stream() { return t_capture ? t_capture : ctx().stream; }
 
 int device_cus()
 {
-    static int cus = 0;
-    if (cus == 0) {
+    static std::atomic<int> cus{0};
+    int c = cus.load(std::memory_order_relaxed);
+    if (c == 0) {
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, ctx().device) == hipSuccess && prop.multiProcessorCount > 0)
-            cus = prop.multiProcessorCount;
+            c = prop.multiProcessorCount;
         else
-            cus = 256;
+            c = 256;
+        cus.store(c, std::memory_order_relaxed);
     }
-    return cus;
+    return c;
 }
 
 int check_launch(const char *what)
@@ -174,6 +189,7 @@ int copy_d2h(void *dst, const void *src, uint64_t bytes)
 }
 
 // --------------------------------------------------------------- profiling --
+// get_event and collect run with prof_mu held.
 static hipEvent_t get_event()
 {
     Context &C = ctx();
@@ -193,7 +209,7 @@ static void collect()
     Context &C = ctx();
     if (C.pending.empty())
         return;
-    (void)hipStreamSynchronize(C.stream);
+    (void)hipStreamSynchronize(C.stream);  // every record is on the library stream
     for (ProfRec &r : C.pending) {
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
@@ -210,24 +226,26 @@ static void collect()
 void prof_begin(int)
 {
     Context &C = ctx();
-    if (!C.prof)
+    if (!C.prof.load(std::memory_order_relaxed) || t_prof_off)
         return;
-    C.open = get_event();
-    if (C.open)
-        (void)hipEventRecord(C.open, C.stream);
+    std::lock_guard<std::mutex> lk(C.prof_mu);
+    t_open = get_event();
+    if (t_open)
+        (void)hipEventRecord(t_open, C.stream);
 }
 
 void prof_end(int kc, double bytes)
 {
     Context &C = ctx();
-    if (!C.prof || !C.open)
+    if (!C.prof.load(std::memory_order_relaxed) || t_prof_off || !t_open)
         return;
+    std::lock_guard<std::mutex> lk(C.prof_mu);
     hipEvent_t b = get_event();
     if (!b)
         return;
     (void)hipEventRecord(b, C.stream);
-    C.pending.push_back({C.open, b, kc, bytes});
-    C.open = nullptr;
+    C.pending.push_back({t_open, b, kc, bytes});
+    t_open = nullptr;
     if (C.pending.size() >= 4096)
         collect();
 }
@@ -235,9 +253,12 @@ void prof_end(int kc, double bytes)
 int64_t tune_value(const char *name, int64_t dflt)
 {
     Context &C = ctx();
-    auto it = C.tune.find(name);
-    if (it != C.tune.end())
-        return it->second;
+    {
+        std::lock_guard<std::mutex> lk(C.tune_mu);
+        auto it = C.tune.find(name);
+        if (it != C.tune.end())
+            return it->second;
+    }
     const std::string env = std::string("DMV_") + name;
     std::string up;
     for (char ch : env)
@@ -435,6 +456,8 @@ void state_destroy(dmv_state *s)
     release(s->ws);
     release(s->cum);
     release(s->scalars);
+    release(s->stage);
+    release(s->terms);
     delete s;
 }
 
@@ -449,18 +472,19 @@ int state_reset(dmv_state *s, uint64_t basis, bool zero)
 
 static const uint64_t kChunk = uint64_t(1) << 24;  // amplitudes per staged transfer
 
-static int staging(double **re, double **im, uint64_t count)
+// The state's own staging buffer for split re/im chunks (allocated on first
+// use, so calls on different states never share one).
+static int staging(dmv_state *s, double **re, double **im, uint64_t count)
 {
-    static double *buf = nullptr;
-    static uint64_t cap = 0;
-    if (count > cap) {
-        release(buf);
-        buf = nullptr;
-        DMV_TRY(alloc((void **)&buf, 2 * count * sizeof(double)));
-        cap = count;
+    if (count > s->stage_cap) {
+        release(s->stage);
+        s->stage = nullptr;
+        s->stage_cap = 0;
+        DMV_TRY(alloc((void **)&s->stage, 2 * count * sizeof(double)));
+        s->stage_cap = count;
     }
-    *re = buf;
-    *im = buf + count;
+    *re = s->stage;
+    *im = s->stage + count;
     return DMV_OK;
 }
 
@@ -470,7 +494,7 @@ int state_get(dmv_state *s, double *re, double *im)
     const uint64_t n_amp = uint64_t(1) << s->n;
     const uint64_t chunk = n_amp < kChunk ? n_amp : kChunk;
     double *dre, *dim;
-    DMV_TRY(staging(&dre, &dim, chunk));
+    DMV_TRY(staging(s, &dre, &dim, chunk));
     for (uint64_t b = 0; b < n_amp; b += chunk) {
         DMV_TRY(launch_split(s->dtype, s->amp, s->layout, b, chunk, dre, dim));
         if (re)
@@ -487,7 +511,7 @@ int state_set(dmv_state *s, const double *re, const double *im)
     const uint64_t n_amp = uint64_t(1) << s->n;
     const uint64_t chunk = n_amp < kChunk ? n_amp : kChunk;
     double *dre, *dim;
-    DMV_TRY(staging(&dre, &dim, chunk));
+    DMV_TRY(staging(s, &dre, &dim, chunk));
     for (uint64_t b = 0; b < n_amp; b += chunk) {
         DMV_TRY(copy_h2d(dre, re + b, chunk * sizeof(double)));
         DMV_TRY(copy_h2d(dim, im + b, chunk * sizeof(double)));
@@ -502,7 +526,7 @@ int state_probs(dmv_state *s, double *host)
     const uint64_t n_amp = uint64_t(1) << s->n;
     const uint64_t chunk = n_amp < kChunk ? n_amp : kChunk;
     double *dre, *dim;
-    DMV_TRY(staging(&dre, &dim, chunk));
+    DMV_TRY(staging(s, &dre, &dim, chunk));
     for (uint64_t b = 0; b < n_amp; b += chunk) {
         prof_begin(DMV_KC_REDUCE);
         DMV_TRY(launch_probs(s->dtype, s->amp, s->layout, b, chunk, dre));
@@ -530,6 +554,8 @@ static int forward_passes(DevicePlan &dp, dmv_state *s)
     return DMV_OK;
 }
 
+static thread_local hipStream_t t_capture_stream = nullptr;
+
 static int forward_graph(DevicePlan &dp, dmv_state *s)
 {
     Context &C = ctx();
@@ -538,13 +564,19 @@ static int forward_graph(DevicePlan &dp, dmv_state *s)
             (void)hipGraphExecDestroy(dp.graph);
             dp.graph = nullptr;
         }
-        const bool prof = C.prof;
-        C.prof = false;  // no event records inside the capture
-        HIPCALL(hipStreamBeginCapture(C.stream, hipStreamCaptureModeThreadLocal));
+        // This thread's launches go to its own capture stream, so that work
+        // other threads put on the library stream meanwhile stays out of the
+        // graph; no event records inside the capture.
+        if (!t_capture_stream)
+            HIPCALL(hipStreamCreateWithFlags(&t_capture_stream, hipStreamNonBlocking));
+        HIPCALL(hipStreamBeginCapture(t_capture_stream, hipStreamCaptureModeThreadLocal));
+        t_capture = t_capture_stream;
+        t_prof_off = true;
         int rc = forward_passes(dp, s);
+        t_capture = nullptr;
+        t_prof_off = false;
         hipGraph_t g = nullptr;
-        hipError_t e = hipStreamEndCapture(C.stream, &g);
-        C.prof = prof;
+        hipError_t e = hipStreamEndCapture(t_capture_stream, &g);
         if (rc != DMV_OK) {
             if (g)
                 (void)hipGraphDestroy(g);
@@ -607,6 +639,7 @@ int circuit_forward(dmv_circuit *c, dmv_state *s, int mode)
 int profile_enable(int on)
 {
     DMV_TRY(ensure_device());
+    std::lock_guard<std::mutex> lk(ctx().prof_mu);
     collect();
     ctx().prof = on != 0;
     return DMV_OK;
@@ -615,6 +648,7 @@ int profile_enable(int on)
 int profile_reset()
 {
     DMV_TRY(ensure_device());
+    std::lock_guard<std::mutex> lk(ctx().prof_mu);
     collect();
     Context &C = ctx();
     for (int i = 0; i < DMV_KC_COUNT; i++) {
@@ -629,6 +663,7 @@ int profile_get(int kc, int64_t *launches, double *ms, double *bytes)
 {
     DMV_CHECK_ARG(kc >= 0 && kc < DMV_KC_COUNT, "unknown kernel class %d", kc);
     DMV_TRY(ensure_device());
+    std::lock_guard<std::mutex> lk(ctx().prof_mu);
     collect();
     Context &C = ctx();
     if (launches)
@@ -643,10 +678,11 @@ int profile_get(int kc, int64_t *launches, double *ms, double *bytes)
 void tune_set(const char *name, int64_t v)
 {
     Context &C = ctx();
+    std::lock_guard<std::mutex> lk(C.tune_mu);
     C.tune[name] = v;
     C.tune_epoch++;
 }
 
-uint64_t tune_epoch() { return ctx().tune_epoch; }
+uint64_t tune_epoch() { return ctx().tune_epoch.load(); }
 
 }  // namespace dmv

@@ -34,6 +34,10 @@ struct dmv_state {
     uint64_t ws_doubles = 0;
     double *cum = nullptr;        // sampler block prefix
     double *scalars = nullptr;    // small device outputs (64 doubles)
+    double *stage = nullptr;      // split re/im staging of getters and setters (on first use)
+    uint64_t stage_cap = 0;       // amplitudes per staged chunk
+    double *terms = nullptr;      // Z/ZZ observable terms of dmv_expect_zz (on first use)
+    uint64_t cap_terms = 0;
 };
 
 namespace dmv {

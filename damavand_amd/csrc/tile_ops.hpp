@@ -437,44 +437,6 @@ __device__ __forceinline__ void tile_body(typename Cplx<R>::T *__restrict__ stat
     store_tile<NT>(v, state, g | thread_global_offset<TLOG>(P->gthr_st, tid), P->greg_st, L);
 }
 
-// A workgroup's share of a pass, software pipelined: while tile t is
-// processed and stored, the loads of the workgroup's next tile are in flight.
-// The prefetch is unconditional (the last iteration re-reads its own tile into
-// the unused buffer), so the compiler's wait counts see the prefetch as the
-// newest loads on every path and the first transpose waits only for tile t.
-// One copy of the phase interpreter keeps the instruction footprint of the
-// non-pipelined kernel.
-template <class R, int RLOG, int TLOG, bool NT>
-__device__ __forceinline__ void tile_stream(typename Cplx<R>::T *__restrict__ state, const DevPass *P,
-                                            const DevPhase *__restrict__ phases, const DevOp *__restrict__ ops,
-                                            uint64_t first, uint64_t stride, uint64_t n_tiles,
-                                            typename Cplx<R>::T *lds, StateLayout L)
-{
-    using V = typename Cplx<R>::T;
-    constexpr int NR = 1 << RLOG, K = RLOG + TLOG;
-    const uint32_t tid = threadIdx.x;
-    const uint64_t tl = thread_global_offset<TLOG>(P->gthr_ld, tid);
-    const uint64_t ts = thread_global_offset<TLOG>(P->gthr_st, tid);
-    V a[NR], b[NR];
-    uint64_t t = first;
-    if (t >= n_tiles)
-        return;
-    load_tile<NT>(a, state, tile_base<K>(P, t) | tl, P->greg_ld, L);
-    for (;;) {
-        const uint64_t t2 = t + stride;
-        const bool more = t2 < n_tiles;
-        load_tile<NT>(b, state, tile_base<K>(P, more ? t2 : t) | tl, P->greg_ld, L);
-        process_tile<R, RLOG, TLOG>(a, P, phases, ops, lds);
-        store_tile<NT>(a, state, tile_base<K>(P, t) | ts, P->greg_st, L);
-        if (!more)
-            break;
-#pragma unroll
-        for (int r = 0; r < NR; r++)
-            a[r] = b[r];
-        t = t2;
-    }
-}
-
 // --------------------------------------------------------------- adjoint --
 // Im<lambda|P|psi> over register pairs (p, p ^ M) for P = X or Y; for Y the
 // sign follows which member holds target bit 0.

@@ -19,7 +19,12 @@
  *    (DMV_OK == 0) with a thread-local message from dmv_last_error().  Host
  *    buffers are owned by the caller and are only used during the call; device
  *    memory is owned by the handle.  One handle must not be used from two
- *    threads at once; different handles may.
+ *    threads at once; different handles may (a call on two states, such as
+ *    dmv_inner_product or dmv_circuit_adjoint, uses both of them).  Each state
+ *    and circuit owns its device buffers, library initialisation and the
+ *    tuning table are locked, and all work is ordered on one library stream.
+ *    Profiling totals taken while several threads launch include the
+ *    other threads' kernels queued between a call's begin and end events.
  */
 #ifndef DAMAVAND_GPU_H
 #define DAMAVAND_GPU_H
@@ -272,7 +277,12 @@ int dmv_circuit_plan_json(dmv_circuit *c, int dtype, int adjoint, int tile_qubit
  * H = sum_q a_q Z_q + sum_{q<n-1} b_q Z_q Z_{q+1} with a_q at
  * coef_offset[i] + q and b_q at coef_offset[i] + n[i] + q.  Outputs:
  * energy[i] and the gradient of its parametrised gates at
- * grad[param_offset[i] + k]. */
+ * grad[param_offset[i] + k].  gate_offset has count + 1 non-decreasing
+ * entries starting at >= 0, coef_offset entries are >= 0, and param_offset
+ * must be the exclusive prefix sum of the circuits' parameter counts
+ * (param_offset[0] = 0), so grad holds sum_i (parameters of circuit i)
+ * values; anything else returns DMV_ERR_ARG before the device is touched.
+ * Circuits have 2 to 24 qubits. */
 int dmv_batch_forward_adjoint(int count, int dtype, const int *n, const int *gate_offset,
                               const int *gate_kind, const int *gate_control,
                               const int *gate_target, const double *gate_param,

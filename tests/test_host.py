@@ -215,3 +215,55 @@ def test_packed_batch_parameter_substitution():
     rows = np.isin(b.kinds, list(wl.PARAMETRIC))
     np.testing.assert_array_equal(b2.params.reshape(-1, 3)[~rows], b.params.reshape(-1, 3)[~rows])
     np.testing.assert_array_equal(b.parameters(), theta0)  # the original is untouched
+
+
+def test_set_parameters_rejects_without_change():
+    """A non-finite value anywhere in the array is rejected before any gate
+    changes, so the parameters (and the cached plans) stay as they were."""
+    gates = [(wl.GATE_RX, -1, 0, (0.1,)), (wl.GATE_RZ, -1, 1, (0.2,)), (wl.GATE_RY, -1, 0, (0.3,))]
+    nc = NativeCircuit(2, gates)
+    v = np.array([1.0, float("nan"), 3.0])
+    with pytest.raises(N.DamavandError, match="non-finite parameter 1"):
+        N.call("dmv_circuit_set_parameters", nc.h, N.dptr(v), 3)
+    out = np.zeros(3)
+    N.call("dmv_circuit_get_parameters", nc.h, N.dptr(out), 3)
+    np.testing.assert_array_equal(out, [0.1, 0.2, 0.3])
+    # values past the parametrised gates are not looked at (circuit.rs:308-322 zips)
+    v = np.array([1.0, 2.0, 3.0, float("inf")])
+    N.call("dmv_circuit_set_parameters", nc.h, N.dptr(v), 4)
+    N.call("dmv_circuit_get_parameters", nc.h, N.dptr(out), 3)
+    np.testing.assert_array_equal(out, [1.0, 2.0, 3.0])
+
+
+def _batch_call(b, **override):
+    import ctypes
+    arrs = dict(n=b.n, gate_offset=b.gate_offset, coef_offset=b.coef_offset, param_offset=b.param_offset)
+    arrs.update({k: np.ascontiguousarray(v, dtype=np.int32) for k, v in override.items()})
+    energy = np.zeros(b.count)
+    grad = np.zeros(max(1, b.total_params))
+    return N.lib().dmv_batch_forward_adjoint(
+        ctypes.c_int(b.count), ctypes.c_int(0), N.iptr(arrs["n"]), N.iptr(arrs["gate_offset"]), N.iptr(b.kinds),
+        N.iptr(b.ctrls), N.iptr(b.tgts), N.dptr(b.params), N.iptr(arrs["coef_offset"]), N.dptr(b.coef),
+        N.iptr(arrs["param_offset"]), N.dptr(energy), N.dptr(grad))
+
+
+def test_batch_rejects_bad_offsets_on_host():
+    """Offsets that would index outside the packed arrays or the gradient are
+    refused before the device is touched (so this runs without a GPU)."""
+    from damavand_amd.batch import pack_circuits
+    b = pack_circuits(wl.batch_workload(4, 79, n_range=(3, 5), layer_range=(1, 2)))
+    po = b.param_offset.copy()
+    po[2] += 1                      # padded: would write past the gradient of the last circuit
+    assert _batch_call(b, param_offset=po) == 1
+    assert "param_offset[2]" in N.last_error()
+    po = b.param_offset.copy()
+    po[1] = -5
+    assert _batch_call(b, param_offset=po) == 1
+    go = b.gate_offset.copy()
+    go[2] = go[1] - 1               # decreasing gate range
+    assert _batch_call(b, gate_offset=go) == 1
+    assert "gate_offset decreases" in N.last_error()
+    co = b.coef_offset.copy()
+    co[3] = -1
+    assert _batch_call(b, coef_offset=co) == 1
+    assert "coef_offset[3]" in N.last_error()

@@ -179,7 +179,8 @@ def test_expect_zz_matches_kron_diagonal():
     gates = _random_gates(rng, n, 50)
     st = oracle.run_gates(n, gates)
     a = rng.normal(size=n)
-    pairs = [(i, i + 1) for i in range(n - 1)] + [(0, 5)]
+    # a reversed pair and a repeated qubit (Z_3 Z_3 = I adds +b) as well
+    pairs = [(i, i + 1) for i in range(n - 1)] + [(0, 5), (5, 0), (3, 3)]
     b = rng.normal(size=len(pairs))
     H = np.zeros((1 << n, 1 << n), dtype=complex)
     for i in range(n):
