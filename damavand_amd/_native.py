@@ -75,6 +75,7 @@ _SIGNATURES = {
     "dmv_inner_product": (ctypes.c_int, [_vp, _vp, _c_dbl_p]),
     "dmv_sample": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_uint64, _c_i64_p]),
     "dmv_sample_uniforms": (ctypes.c_int, [_vp, ctypes.c_int64, _c_dbl_p, _c_i64_p]),
+    "dmv_sample_static": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_uint64, _c_i64_p]),
     "dmv_circuit_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp)]),
     "dmv_circuit_destroy": (ctypes.c_int, [_vp]),
     "dmv_circuit_add_gate": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_dbl_p, ctypes.c_int,

@@ -22,6 +22,29 @@ int batch_forward_adjoint(int count, int dtype, const int *n, const int *gate_of
 int autotune(dmv_circuit *c, dmv_state *s, int *chosen);
 }  // namespace dmv
 
+namespace dmv {
+// k samples of s into the device array d_out (uniforms from d_u, or from the
+// seeded stream when d_u is null).  Uses only the state's own workspace
+// (block masses and their prefix) and allocates nothing.
+int sample_to_device(dmv_state *s, int64_t k, uint64_t seed, const double *d_u, int64_t *d_out)
+{
+    // the reference panics when it cannot sample (circuit.rs:478-482): refuse a zero state
+    const int bb = s->n < 12 ? s->n : 12;
+    DMV_TRY(launch_block_norms(s->dtype, s->amp, s->n, s->layout, bb, s->ws));
+    DMV_TRY(launch_sum(s->ws, uint64_t(1) << (s->n - bb), s->scalars));
+    double total = 0.0;
+    DMV_TRY(copy_d2h(&total, s->scalars, sizeof(double)));
+    if (!(total > 0.0 && std::isfinite(total))) {
+        set_error("could not sample from output distribution (total probability %g)", total);
+        return DMV_ERR_STATE;
+    }
+    prof_begin(DMV_KC_SAMPLE);
+    DMV_TRY(launch_sample(s->dtype, s->amp, s->n, s->layout, k, seed, d_u, s->cum, s->ws, d_out));
+    prof_end(DMV_KC_SAMPLE, (double)s->bytes);
+    return DMV_OK;
+}
+}  // namespace dmv
+
 using namespace dmv;
 
 #define CHECK_STATE(s) DMV_CHECK_ARG((s) != nullptr && (s)->amp != nullptr, "null state")
@@ -342,41 +365,21 @@ static int sample_impl(dmv_state *s, int64_t k, uint64_t seed, const double *u_h
     DMV_CHECK_ARG(k >= 0, "negative sample count");
     if (k == 0)
         return DMV_OK;
+    if (u_host)
+        for (int64_t i = 0; i < k; i++)
+            DMV_CHECK_ARG(u_host[i] > 0.0 && u_host[i] <= 1.0, "uniform %lld is not in (0, 1]", (long long)i);
     DMV_TRY(ensure_device());
     int64_t *d_out = nullptr;
     double *d_u = nullptr;
     DMV_TRY(alloc((void **)&d_out, (uint64_t)k * sizeof(int64_t)));
     int rc = DMV_OK;
     if (u_host) {
-        for (int64_t i = 0; i < k && rc == DMV_OK; i++)
-            if (!(u_host[i] > 0.0 && u_host[i] <= 1.0)) {
-                set_error("uniform %lld is not in (0, 1]", (long long)i);
-                rc = DMV_ERR_ARG;
-            }
-        if (rc == DMV_OK)
-            rc = alloc((void **)&d_u, (uint64_t)k * sizeof(double));
+        rc = alloc((void **)&d_u, (uint64_t)k * sizeof(double));
         if (rc == DMV_OK)
             rc = copy_h2d(d_u, u_host, (uint64_t)k * sizeof(double));
     }
-    if (rc == DMV_OK) {
-        // the reference panics when it cannot sample (circuit.rs:478-482): refuse a zero state
-        const int bb = s->n < 12 ? s->n : 12;
-        rc = launch_block_norms(s->dtype, s->amp, s->n, s->layout, bb, s->ws);
-        if (rc == DMV_OK)
-            rc = launch_sum(s->ws, uint64_t(1) << (s->n - bb), s->scalars);
-        double total = 0.0;
-        if (rc == DMV_OK)
-            rc = copy_d2h(&total, s->scalars, sizeof(double));
-        if (rc == DMV_OK && !(total > 0.0 && std::isfinite(total))) {
-            set_error("could not sample from output distribution (total probability %g)", total);
-            rc = DMV_ERR_STATE;
-        }
-    }
-    if (rc == DMV_OK) {
-        prof_begin(DMV_KC_SAMPLE);
-        rc = launch_sample(s->dtype, s->amp, s->n, s->layout, k, seed, d_u, s->cum, s->ws, d_out);
-        prof_end(DMV_KC_SAMPLE, (double)s->bytes);
-    }
+    if (rc == DMV_OK)
+        rc = sample_to_device(s, k, seed, d_u, d_out);
     if (rc == DMV_OK)
         rc = copy_d2h(out, d_out, (uint64_t)k * sizeof(int64_t));
     release(d_out);

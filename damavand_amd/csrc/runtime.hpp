@@ -114,4 +114,8 @@ int state_probs(dmv_state *s, double *host);
 
 int circuit_forward(dmv_circuit *c, dmv_state *s, int mode);
 
+// k samples of s into the device array d_out (uniforms d_u, or the seeded
+// stream when d_u is null), using only the state's workspace: no allocation.
+int sample_to_device(dmv_state *s, int64_t k, uint64_t seed, const double *d_u, int64_t *d_out);
+
 }  // namespace dmv

@@ -212,6 +212,17 @@ int dmv_inner_product(dmv_state *a, dmv_state *b, double *out_re_im);
 int dmv_sample(dmv_state *s, int64_t k, uint64_t seed, int64_t *out);
 /* The same with caller-given uniforms in (0, 1]. */
 int dmv_sample_uniforms(dmv_state *s, int64_t k, const double *u, int64_t *out);
+/* dmv_sample without any device allocation (no hipMalloc or hipFree, no
+ * allocating library call): the same samples for the same state, k and seed.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * (code under stream capture, serving loops, harnesses that forbid it).
+ * The k results are staged in a fixed device array of 2^20 int64 (8 MiB,
+ * held on each GPU used until the process ends); k > 2^20 returns
+ * DMV_ERR_ARG.  Block masses and their prefix use the state's own workspace,
+ * allocated by dmv_state_create.  All calls on one GPU share the staging
+ * array, which keeps the last call's results; the library serialises these
+ * calls with a lock, and calls from other processes have their own copy. */
+int dmv_sample_static(dmv_state *s, int64_t k, uint64_t seed, int64_t *out);
 
 /* ---- circuits (gate list + fused execution plan) ---- */
 int dmv_circuit_create(int num_qubits, dmv_circuit **out);
