@@ -57,3 +57,38 @@ class NativeCircuit:
         N.call("dmv_circuit_plan_stats", self.h, dtype, ctypes.byref(p), ctypes.byref(ph), ctypes.byref(o),
                ctypes.byref(b))
         return p.value, ph.value, o.value, b.value
+
+
+def _hip_runtime():
+    """The HIP runtime instance the library itself loaded (found in this
+    process's mappings), so that raw copies see the library's allocations."""
+    N.lib()
+    path = next(line.split()[-1] for line in open("/proc/self/maps")
+                if "libamdhip64.so" in line and "/torch/" not in line)
+    hip = ctypes.CDLL(path)
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipMemcpy.restype = ctypes.c_int
+    return hip
+
+
+def amplitudes_at(state_handle, n, indices, amp_bytes=16):
+    """Amplitudes of a device state at the given plain indices, read element
+    by element from its buffer through the layout that dmv_state_layout
+    reports (include/damavand_gpu.h), without copying the whole state."""
+    from tests.tile_emulator import element_map
+    ptr = ctypes.c_void_p()
+    N.call("dmv_state_device_ptr", state_handle, ctypes.byref(ptr), None)
+    sh, bb, fo = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+    N.call("dmv_state_layout", state_handle, ctypes.byref(sh), ctypes.byref(bb), ctypes.byref(fo))
+    idx = np.asarray(indices, dtype=np.int64)
+    assert idx.min() >= 0 and idx.max() < (1 << n)
+    el = element_map(idx, sh.value, bb.value, fo.value)
+    hip = _hip_runtime()
+    N.synchronize()
+    dt = np.complex128 if amp_bytes == 16 else np.complex64
+    out = np.empty(idx.size, dtype=dt)
+    one = np.empty(1, dtype=dt)
+    for j, e in enumerate(el):
+        assert hip.hipMemcpy(one.ctypes.data, ptr.value + int(e) * amp_bytes, amp_bytes, 2) == 0
+        out[j] = one[0]
+    return out.astype(np.complex128)
