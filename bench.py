@@ -11,10 +11,20 @@ state (16 GiB) that stays resident in HBM.  Prints one JSON line:
 * roofline: the fused tile kernel's achieved GB/s from its own HIP-event
   launch durations (same stream as the launches) against the 8 TB/s peak;
 * cpu_baseline: the oracle (C/OpenMP restatement of the reference's
-  multithreading path) timed on this host on a bounded sample of the same
-  workload;
+  multithreading path) on this host: the same 30-qubit forward timed on a
+  bounded sample of its gates and extrapolated (value = the GPU metric's
+  bytes over the CPU time per forward), the 20-qubit CPU workload end to end
+  (990 gates, probabilities, 1000 samples with the reference sampler, <Z_i>)
+  on all threads and on one, the 24-qubit parameter shift estimated from a
+  timed sample, and oracle energies of 14 batched circuits;
 * adjoint: the 24-qubit, 960-parameter ansatz gradient by the adjoint
-  kernels versus parameter shift with plain fused forwards, on the same GPU.
+  kernels versus parameter shift with plain fused forwards on the same GPU,
+  against the oracle's parameter shift (fixture) and central differences;
+* config1_gpu: the 20-qubit workload on the GPU with its errors against
+  the oracle's results;
+* batch: 4096 circuits of 12-18 qubits, forward + adjoint in one launch per
+  dtype (complex128, complex64), energies against the oracle;
+* stress: the 33-qubit complex128 circuit (128 GiB), <Z_i> and 10^5 samples.
 """
 from __future__ import annotations
 
@@ -148,19 +158,46 @@ def bench_adjoint(n, layers, shift_params):
     c.set_parameters(theta)
     per_param = t_ps / idx.size
     diff = float(np.max(np.abs(g[idx] - ps)))
+    # central finite differences of the GPU energy, h = 1e-6, on 8 seeded parameters
+    h = 1e-6
+    fd_idx = np.sort(rng.choice(theta.size, size=8, replace=False))
+    fd_err = 0.0
+    for k in fd_idx:
+        vals = []
+        for s in (h, -h):
+            th = theta.copy()
+            th[k] += s
+            c.set_parameters(th)
+            c.reset_amplitudes()
+            c.forward()
+            vals.append(c.expectation(a, pairs, b))
+        fd_err = max(fd_err, abs((vals[0] - vals[1]) / (2 * h) - g[k]))
+    c.set_parameters(theta)
+    # the oracle's parameter shift (tests/golden/config3_oracle.npz, written by
+    # tests/golden/make_config3.py) on its 16 seeded parameters
+    oracle_diff = oracle_energy_diff = None
+    fx = os.path.join(ROOT, "tests", "golden", "config3_oracle.npz")
+    if os.path.exists(fx) and (n, layers) == (24, 20):
+        ref = np.load(fx)
+        oracle_diff = float(np.max(np.abs(g[ref["idx"]] - ref["grad_shift"])))
+        oracle_energy_diff = abs(e - float(ref["energy"]))
     log(f"[bench] adjoint {n}q {theta.size} params: {t_adj * 1e3:.1f} ms; parameter shift "
-        f"{per_param * 1e3:.2f} ms/param; max |adjoint - shift| {diff:.2e}")
+        f"{per_param * 1e3:.2f} ms/param; max |adjoint - shift| {diff:.2e}, vs oracle shift {oracle_diff}, "
+        f"vs FD {fd_err:.2e}")
     return {
         "workload": f"{n}-qubit RY/RZ + CNOT-ring ansatz, {layers} layers, {theta.size} parameters, Z/ZZ chain H",
         "adjoint_s": t_adj, "adjoint_grads_per_s": theta.size / t_adj,
         "param_shift_s_per_param": per_param, "param_shift_grads_per_s": 1.0 / per_param,
         "param_shift_sampled_params": int(idx.size),
         "param_shift_full_s_extrapolated": per_param * theta.size,
-        "speedup": (per_param * theta.size) / t_adj, "max_abs_diff_vs_shift": diff, "energy": e,
+        "speedup": (per_param * theta.size) / t_adj, "max_abs_diff_vs_shift": diff,
+        "max_abs_diff_vs_oracle_shift": oracle_diff, "oracle_shift_params": 16 if oracle_diff is not None else 0,
+        "abs_diff_energy_vs_oracle": oracle_energy_diff,
+        "max_abs_diff_vs_fd": fd_err, "fd_h": h, "fd_params": int(fd_idx.size), "energy": e,
     }
 
 
-def bench_batch(count):
+def bench_batch(count, oracle_energy=None):
     """4096 ansatz circuits (12-18 qubits, 4-12 layers), forward + adjoint, one launch each dtype."""
     from damavand_amd import _native as N
     from damavand_amd.batch import batch_forward_adjoint_packed, pack_circuits
@@ -190,6 +227,14 @@ def bench_batch(count):
     e64, g64 = res["complex64"]
     out["c64_max_abs_err_energy_vs_c128"] = float(np.max(np.abs(e64 - e128)))
     out["c64_max_abs_err_grad_vs_c128"] = float(np.max(np.abs(g64 - g128)))
+    if oracle_energy:
+        ids = sorted(oracle_energy)
+        r = np.array([oracle_energy[i] for i in ids])
+        out["energy_vs_oracle"] = {"circuits": len(ids),
+                                   "c128_max_abs_err": float(np.max(np.abs(e128[ids] - r))),
+                                   "c64_max_abs_err": float(np.max(np.abs(e64[ids] - r)))}
+        log(f"[bench] batch energies vs oracle on {len(ids)} circuits: c128 "
+            f"{out['energy_vs_oracle']['c128_max_abs_err']:.2e}, c64 {out['energy_vs_oracle']['c64_max_abs_err']:.2e}")
     return out
 
 
@@ -227,40 +272,187 @@ def bench_stress(n, layers, samples):
             "distinct_samples": int(np.unique(s).size)}
 
 
-def bench_cpu_baseline(n, layers, sample_gates):
-    """Oracle (port of the reference's multithreading path) on this host."""
-    import oracle
-    from damavand_amd import workloads as wl
-    gates = wl.layered_circuit(n, layers, 1002)[:sample_gates]
-    t0 = time.perf_counter()
-    st = oracle.OracleState(n)
-    t_init = time.perf_counter() - t0
+def host_info():
+    """CPU model, logical cores and RAM of this host (from /proc)."""
+    model, ram_gib = "unknown", None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+        with open("/proc/meminfo") as f:
+            for line in f:
+                if line.startswith("MemTotal"):
+                    ram_gib = round(int(line.split()[1]) / 2 ** 20, 1)
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "logical_cpus": os.cpu_count(), "ram_gib": ram_gib}
+
+
+def _timed_gates(st, gates):
     t0 = time.perf_counter()
     for kind, ctl, tgt, p in gates:
         st.apply_gate(kind, tgt, ctl, p)
-    t = time.perf_counter() - t0
-    algo = sum(wl.algorithmic_bytes_per_gate(g, n) for g in gates)
+    return time.perf_counter() - t0
+
+
+def bench_cpu_baseline(n, layers, sample_gates, gpu_hbm_bytes, gpu_ms_step):
+    """The oracle (C/OpenMP port of the reference's multithreading path,
+    circuit_multithreading.rs:9-54: full clone + one pass per gate) on this
+    host, timed on `sample_gates` gates spread evenly over the n-qubit layered
+    circuit and extrapolated to one forward of the whole circuit.  `value` is
+    the GPU metric's own numerator (algorithmic bytes of the fused forward)
+    over that CPU time per forward, so CPU and GPU values divide the same
+    work by each side's time for it."""
+    import oracle
+    from damavand_amd import workloads as wl
+    full = wl.layered_circuit(n, layers, 1002)
+    stride = max(1, len(full) // sample_gates)
+    gates = full[::stride][:sample_gates]
+    t0 = time.perf_counter()
+    st = oracle.OracleState(n)
+    t_init = time.perf_counter() - t0
+    t = _timed_gates(st, gates)
+    per_gate = t / len(gates)
     cores = oracle.num_threads()
-    # one more gate on a single thread (the reference's per-gate pass without the thread pool)
+    s_forward = per_gate * len(full)
+    algo = sum(wl.algorithmic_bytes_per_gate(g, n) for g in gates)
+    # one more gate on a single thread
     oracle.set_num_threads(1)
-    g1 = wl.layered_circuit(n, layers, 1002)[sample_gates]
-    t0 = time.perf_counter()
-    st.apply_gate(g1[0], g1[2], g1[1], g1[3])
-    t1 = time.perf_counter() - t0
+    t1 = _timed_gates(st, [full[1]])
     oracle.set_num_threads(cores)
-    # the reference sampler (utils.rs:258-277): one (N+1)-long cumulative rebuilt per draw
-    probs = st.probabilities()
+    del st
+    log(f"[bench] cpu {n}q: {per_gate:.2f} s/gate on {cores} threads -> {s_forward:.0f} s per forward "
+        f"(GPU {gpu_ms_step / 1e3:.3f} s)")
+    return {"value": gpu_hbm_bytes / s_forward / 1e9, "unit": "GB/s", "cores": cores, "kind": "port",
+            "sample": f"{len(gates)} gates spread over the {n}-qubit layered circuit ({len(full)} gates), "
+                      f"complex128, {t:.1f} s on {cores} threads ({per_gate:.2f} s/gate, state init {t_init:.1f} s), "
+                      f"extrapolated to one forward",
+            "s_per_forward": s_forward, "gpu_s_per_forward": gpu_ms_step / 1e3,
+            "gpu_speedup_per_forward": s_forward / (gpu_ms_step / 1e3),
+            "per_gate_GBps": algo / t / 1e9,
+            "per_gate_GBps_1_thread": wl.algorithmic_bytes_per_gate(full[1], n) / t1 / 1e9,
+            "host": host_info()}
+
+
+def bench_config1_cpu(n, layers, samples):
+    """The CPU workload end to end on the oracle: forward of the 990-gate
+    20-qubit circuit, probabilities, `samples` draws with the reference's
+    sampler (an (N+1)-long cumulative rebuilt per draw, src/utils.rs:258-277)
+    and <Z_i>, on all threads and on one."""
+    import oracle
+    from damavand_amd import workloads as wl
+    gates = wl.layered_circuit(n, layers, 1001)
+    u = wl.sample_uniforms(1001, samples)
+    cores = oracle.num_threads()
+    out = {"workload": f"{n}-qubit complex128 layered circuit, {layers} layers ({len(gates)} gates), "
+                       f"probabilities, {samples} samples, <Z_i>", "cores": cores}
+    res = None
+    for threads in (cores, 1):
+        oracle.set_num_threads(threads)
+        t0 = time.perf_counter()
+        st = oracle.OracleState(n)
+        _timed_gates(st, gates)
+        t_fwd = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        prob = st.probabilities()
+        t_prob = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        s = oracle.sample_uniforms(prob, u, rebuild=True)
+        t_samp = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        ez = st.expect_z()
+        t_z = time.perf_counter() - t0
+        key = "all_threads" if threads == cores else "one_thread"
+        out[key] = {"threads": threads, "forward_s": t_fwd, "probabilities_s": t_prob, "sample_s": t_samp,
+                    "expect_z_s": t_z, "total_s": t_fwd + t_prob + t_samp + t_z}
+        if res is None:
+            res = (st.amp.copy(), prob, s, ez)
+        del st
+    oracle.set_num_threads(cores)
+    log(f"[bench] cpu 20q workload: {out['all_threads']['total_s']:.2f} s on {cores} threads, "
+        f"{out['one_thread']['total_s']:.2f} s on one")
+    return out, res
+
+
+def bench_config1_gpu(n, layers, samples, cpu_result):
+    """The same workload on the GPU, timed, and checked against the oracle's
+    results of the CPU leg."""
+    from damavand_amd import workloads as wl
+    gates = wl.layered_circuit(n, layers, 1001)
+    u = wl.sample_uniforms(1001, samples)
+    c = build_circuit(n, gates)
+    c.forward()
+    c.measure()
+    sync()
+    c.reset_amplitudes()
+    sync()
     t0 = time.perf_counter()
-    oracle.sample_uniforms(probs, np.array([0.25, 0.75]), rebuild=True)
-    t_sample = (time.perf_counter() - t0) / 2
-    del st, probs
-    return {"value": algo / t / 1e9, "unit": "GB/s", "cores": cores, "kind": "port",
-            "sample": f"first {len(gates)} gates (H on qubits 0..{len(gates) - 1}) of the {n}-qubit layered "
-                      f"circuit, complex128, {t:.1f} s ({t / len(gates):.2f} s/gate; state init {t_init:.1f} s)",
-            "value_1_thread": wl.algorithmic_bytes_per_gate(g1, n) / t1 / 1e9,
-            "sample_1_thread": f"gate {sample_gates} of the same circuit on one thread, {t1:.1f} s",
-            "sampler_s_per_draw": t_sample,
-            "sampler_note": f"reference sampler at {n} qubits, cumulative rebuilt per draw, mean of 2 draws"}
+    c.forward()
+    sync()
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    prob = c.measure()
+    t_prob = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    s = c.state.sample_uniforms(u)
+    t_samp = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ez = c.expectation_z()
+    t_z = time.perf_counter() - t0
+    out = {"forward_s": t_fwd, "probabilities_s": t_prob, "sample_s": t_samp, "expect_z_s": t_z,
+           "total_s": t_fwd + t_prob + t_samp + t_z}
+    if cpu_result is not None:
+        amp, p_ref, s_ref, ez_ref = cpu_result
+        out["max_abs_err_amplitudes"] = float(np.max(np.abs(c.retrieve_amplitudes_on_host() - amp)))
+        out["max_abs_err_probabilities"] = float(np.max(np.abs(prob - p_ref)))
+        out["max_abs_err_expect_z"] = float(np.max(np.abs(ez - ez_ref)))
+        out["samples_equal"] = int(np.sum(s == s_ref))
+    log(f"[bench] gpu 20q workload: {out['total_s'] * 1e3:.1f} ms")
+    return out
+
+
+def bench_adjoint_cpu_shift(n, layers, sample_gates):
+    """The oracle's parameter shift for the 24-qubit ansatz, timed on a
+    bounded sample: `sample_gates` gates of its forward, extrapolated to one
+    forward and to two forwards per parameter."""
+    import oracle
+    from damavand_amd import workloads as wl
+    gates, a, b = wl.ansatz_circuit(n, layers, 1003)
+    st = oracle.OracleState(n)
+    t = _timed_gates(st, gates[:sample_gates])
+    del st
+    s_forward = t / sample_gates * len(gates)
+    n_params = wl.count_parameters(gates)
+    log(f"[bench] cpu {n}q ansatz: {s_forward:.1f} s per forward -> parameter shift "
+        f"{2 * s_forward * n_params:.0f} s for {n_params} parameters")
+    return {"cpu_forward_s": s_forward, "cpu_param_shift_s_per_param": 2 * s_forward,
+            "cpu_param_shift_full_s": 2 * s_forward * n_params, "cores": oracle.num_threads(),
+            "sample": f"first {sample_gates} of {len(gates)} gates timed, extrapolated"}
+
+
+def bench_batch_cpu(count, per_qubits):
+    """Oracle energies of `per_qubits` circuits per qubit count of the batched
+    workload: the CPU cost per circuit forward, and the FP64 reference values
+    that the GPU batch's complex128 and complex64 energies are compared with."""
+    import oracle
+    from damavand_amd import workloads as wl
+    circuits = wl.batch_workload(count, 1004)
+    picks = []
+    for q in range(12, 19):
+        picks += [i for i, cc in enumerate(circuits) if cc[0] == q][:per_qubits]
+    ref = {}
+    t0 = time.perf_counter()
+    for i in picks:
+        n, gates, a, b = circuits[i]
+        pairs = wl.chain_pairs(n)
+        st = oracle.run_gates(n, gates)
+        ref[i] = st.expect_zz(a, pairs, b)
+    t = time.perf_counter() - t0
+    return {"circuits": len(picks), "cpu_energy_s": t, "cpu_s_per_circuit_forward": t / len(picks),
+            "cores": oracle.num_threads()}, ref
 
 
 def measured_traffic(qubits, layers):
@@ -290,12 +482,16 @@ def main():
     ap.add_argument("--adjoint-qubits", type=int, default=24)
     ap.add_argument("--adjoint-layers", type=int, default=20)
     ap.add_argument("--shift-params", type=int, default=96)
-    ap.add_argument("--cpu-gates", type=int, default=6)
+    ap.add_argument("--cpu-gates", type=int, default=12,
+                    help="gates of the 30-qubit circuit the CPU baseline times (spread over the circuit)")
+    ap.add_argument("--cpu-adjoint-gates", type=int, default=144,
+                    help="gates of the 24-qubit ansatz the CPU parameter-shift estimate times")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-adjoint", action="store_true")
     ap.add_argument("--skip-per-gate", action="store_true")
-    ap.add_argument("--batch", type=int, default=0, help="also run the batched workload with this many circuits")
-    ap.add_argument("--stress", type=int, default=0, help="also run the stress workload at this many qubits")
+    ap.add_argument("--skip-config1", action="store_true")
+    ap.add_argument("--batch", type=int, default=4096, help="circuits of the batched workload (0: skip)")
+    ap.add_argument("--stress", type=int, default=33, help="qubits of the stress workload (0: skip)")
     ap.add_argument("--stress-layers", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -313,11 +509,29 @@ def main():
         log(f"[bench] per-gate mode: {per_gate['ms']:.0f} ms, kernels {per_gate['GBps']:.0f} GB/s")
     del circ
     adj = None if args.skip_adjoint else bench_adjoint(args.adjoint_qubits, args.adjoint_layers, args.shift_params)
-    batch = bench_batch(args.batch) if args.batch else None
+    # the 20-qubit CPU workload: the oracle end to end, then the GPU on the same circuit
+    cfg1_cpu = cfg1_res = None
+    if not args.skip_config1 and not args.skip_cpu:
+        cfg1_cpu, cfg1_res = bench_config1_cpu(20, 10, 1000)
+    cfg1_gpu = None if args.skip_config1 else bench_config1_gpu(20, 10, 1000, cfg1_res)
+    batch_cpu = batch_ref = None
+    if args.batch and not args.skip_cpu:
+        batch_cpu, batch_ref = bench_batch_cpu(args.batch, 2)
+    batch = bench_batch(args.batch, batch_ref) if args.batch else None
     stress = bench_stress(args.stress, args.stress_layers, 100000) if args.stress else None
-    cpu = None if args.skip_cpu else bench_cpu_baseline(args.qubits, args.layers, args.cpu_gates)
-    if cpu:
-        log(f"[bench] cpu baseline {cpu['value']:.2f} GB/s on {cpu['cores']} threads")
+    cpu = None
+    if not args.skip_cpu:
+        cpu = bench_cpu_baseline(args.qubits, args.layers, args.cpu_gates, fw["hbm_bytes"], fw["ms_step"])
+        if cfg1_cpu:
+            cpu["config1"] = cfg1_cpu
+        if adj:
+            cpu["adjoint_param_shift"] = bench_adjoint_cpu_shift(args.adjoint_qubits, args.adjoint_layers,
+                                                                 args.cpu_adjoint_gates)
+            adj["cpu_param_shift_full_s"] = cpu["adjoint_param_shift"]["cpu_param_shift_full_s"]
+            adj["speedup_vs_cpu_param_shift"] = adj["cpu_param_shift_full_s"] / adj["adjoint_s"]
+        if batch_cpu:
+            cpu["batch"] = batch_cpu
+        log(f"[bench] cpu baseline {cpu['value']:.2f} GB/s (forward-equivalent) on {cpu['cores']} threads")
 
     value = fw["hbm_bytes"] / (fw["ms_step"] * 1e6)
     out = {
@@ -350,6 +564,7 @@ def main():
         "copy_ceiling_GBps": round(copy_gbps, 1),
         "per_gate_mode": per_gate,
         "adjoint": adj,
+        "config1_gpu": cfg1_gpu,
     }
     if batch:
         out["batch"] = batch

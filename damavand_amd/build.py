@@ -33,6 +33,11 @@ COMMON = [
     # their common code merges register-array accesses into pointer phis,
     # which sends the tile's registers to scratch memory
     "-mllvm", "-simplifycfg-sink-common=false",
+    # and keep them from being folded into select chains: with 8 registers
+    # per thread the pair ops' small cases were otherwise speculated and
+    # merged, indexing the register array by the runtime mask (a 16-way
+    # compare-and-select per amplitude, twice the instructions of the op)
+    "-mllvm", "-two-entry-phi-node-folding-threshold=0",
     "-I", os.path.join(ROOT, "include"),
 ]
 

@@ -566,15 +566,30 @@ int dmv_circuit_adjoint(dmv_circuit *c, dmv_state *psi, dmv_state *lam, const do
     // reverse sweep
     DMV_TRY(prepare_plan(c->adj, c->version, c->n, psi->dtype, true, adjoint_ops(c->gates), psi->layout));
     DevicePlan &dp = c->adj;
-    // partials per slot: one per wave of every tile (tile path), one (whole-state path)
-    const uint64_t n_tiles = dp.small ? 1 : (uint64_t(1) << (c->n - dp.pp.k)) * ((uint64_t(1) << dp.pp.tlog) / 64);
-    const int n_slots = dp.plan.n_slots;
-    const uint64_t need = (uint64_t)(n_slots > 0 ? n_slots : 1) * n_tiles;
-    if (need > c->cap_partials) {
+    if (dp.plan.n_slots != np) {
+        set_error("internal: adjoint plan has %d slots for %d parameters", dp.plan.n_slots, np);
+        return DMV_ERR_STATE;
+    }
+    // gradient partials: per (IP op, tile, wave) on the tile path, per slot
+    // on the whole-state path.  The tile path keeps the rows of as many
+    // consecutive passes as fit in kPartialBudget bytes (at least one pass)
+    // and sums them into grad with one launch whenever the next pass would
+    // not fit, so the buffer stays bounded at any state size.
+    constexpr uint64_t kPartialBudget = uint64_t(256) << 20;
+    uint64_t per_ip = 1, rows = (uint64_t)(np > 0 ? np : 1);
+    if (!dp.small) {
+        per_ip = (uint64_t(1) << (c->n - dp.pp.k)) * ((uint64_t(1) << dp.pp.tlog) / 64);
+        uint64_t widest = 1;
+        for (size_t p = 0; p < dp.plan.passes.size(); p++)
+            widest = std::max<uint64_t>(widest, dp.plan.ip_begin[p + 1] - dp.plan.ip_begin[p]);
+        rows = std::max(widest, std::min<uint64_t>(std::max<uint64_t>(1, dp.plan.ip_slots.size()),
+                                                   kPartialBudget / (per_ip * sizeof(double))));
+    }
+    if (rows * per_ip > c->cap_partials) {
         release(c->d_partials);
         c->d_partials = nullptr;
-        DMV_TRY(alloc((void **)&c->d_partials, need * sizeof(double)));
-        c->cap_partials = need;
+        DMV_TRY(alloc((void **)&c->d_partials, rows * per_ip * sizeof(double)));
+        c->cap_partials = rows * per_ip;
     }
     if ((uint64_t)np + 1 > c->cap_grad) {
         release(c->d_grad);
@@ -587,19 +602,23 @@ int dmv_circuit_adjoint(dmv_circuit *c, dmv_state *psi, dmv_state *lam, const do
         DMV_TRY(launch_small_adjoint(psi->dtype, psi->amp, lam->amp, c->n, dp.d_ops, (uint32_t)dp.small_ops.size(),
                                      c->d_partials));
         prof_end(DMV_KC_ADJOINT, 4.0 * (double)psi->bytes);
+        DMV_TRY(launch_slot_sums(c->d_partials, np, 1, nullptr, c->d_grad));
     } else {
-        for (uint32_t p = 0; p < dp.plan.passes.size(); p++) {
+        const uint32_t n_pass = (uint32_t)dp.plan.passes.size();
+        uint32_t pending = 0;  // first IP row not yet summed
+        for (uint32_t p = 0; p < n_pass; p++) {
+            const uint64_t row0 = dp.plan.ip_begin[p] - pending;
             prof_begin(DMV_KC_ADJOINT);
             DMV_TRY(launch_tile_adjoint(psi->dtype, dp.pp.rlog, dp.pp.tlog, psi->amp, lam->amp, c->n, psi->layout, dp.d_passes,
-                                        dp.d_phases, dp.d_ops, p, c->d_partials));
+                                        dp.d_phases, dp.d_ops, p, c->d_partials + row0 * per_ip));
             prof_end(DMV_KC_ADJOINT, 4.0 * (double)psi->bytes);
+            const uint32_t done = dp.plan.ip_begin[p + 1];
+            if (p + 1 == n_pass || dp.plan.ip_begin[p + 2] - pending > rows) {
+                DMV_TRY(launch_slot_sums(c->d_partials, (int)(done - pending), per_ip, dp.d_ip_slots + pending, c->d_grad));
+                pending = done;
+            }
         }
     }
-    if (n_slots != np) {
-        set_error("internal: adjoint plan has %d slots for %d parameters", n_slots, np);
-        return DMV_ERR_STATE;
-    }
-    DMV_TRY(launch_slot_sums(c->d_partials, n_slots, n_tiles, c->d_grad));
     if (energy)
         DMV_TRY(copy_d2h(energy, psi->scalars, sizeof(double)));
     if (grad && np > 0)

@@ -97,7 +97,7 @@ struct alignas(16) DevOp {
     uint32_t ctrl_thr;
     uint64_t ctrl;     // every bit of ctrl must be 1 for the op to act
     uint32_t plain;
-    uint32_t pad1;
+    uint32_t ip_pos;   // IP op of a tile plan: its index among the IP ops of its pass
     double m[8];       // re/im of m00, m01, m10, m11
 };
 static_assert(sizeof(DevOp) == 112, "DevOp layout");

@@ -11,12 +11,12 @@ struct GradSink {
     static constexpr bool kPerWave = false;
     double *grad;
     bool first;  // first tile of the pass: store, later tiles: accumulate
-    __device__ void operator()(int slot, double v, uint32_t) const
+    __device__ void operator()(const DevOp *op, double v, uint32_t) const
     {
         if (first)
-            grad[slot] = v;
+            grad[op->slot] = v;
         else
-            grad[slot] += v;
+            grad[op->slot] += v;
     }
 };
 
@@ -157,10 +157,13 @@ static int run_batch(const BatchLaunch &L)
     return check_launch("batch kernel");
 }
 
+// 256 threads and 8 amplitudes per thread and state (k = 11): the shape of
+// the reverse-sweep passes, which hold psi and lambda in registers together.
 int batch_tile_shape(int dtype, int *rlog, int *tlog)
 {
+    (void)dtype;
     *tlog = 8;
-    *rlog = dtype == DMV_C64 ? 5 : 4;
+    *rlog = 3;
     return DMV_OK;
 }
 
@@ -168,8 +171,8 @@ int launch_batch(int dtype, const BatchLaunch &L)
 {
     DMV_CHECK_ARG(L.grid >= 1 && L.count >= 1, "empty batch");
     if (dtype == DMV_C64)
-        return run_batch<float, 5, 8>(L);
-    return run_batch<double, 4, 8>(L);
+        return run_batch<float, 3, 8>(L);
+    return run_batch<double, 3, 8>(L);
 }
 
 }  // namespace dmv

@@ -468,8 +468,10 @@ int launch_inner(int dtype, const void *a, const void *b, int n, StateLayout L, 
     return check_launch("inner product final");
 }
 
-// One workgroup per slot: out[s] = sum_t partials[s * n_tiles + t].
-__global__ void __launch_bounds__(RT) k_slot_sums(const double *__restrict__ partials, uint64_t n_tiles, double *out)
+// One workgroup per row b: out[slot_map ? slot_map[b] : b] = sum_t
+// partials[b * n_tiles + t], in a fixed order.
+__global__ void __launch_bounds__(RT) k_slot_sums(const double *__restrict__ partials, uint64_t n_tiles,
+                                                  const int *__restrict__ slot_map, double *out)
 {
     __shared__ double red[RT / 64];
     const double *p = partials + (uint64_t)blockIdx.x * n_tiles;
@@ -478,14 +480,14 @@ __global__ void __launch_bounds__(RT) k_slot_sums(const double *__restrict__ par
         acc += p[t];
     acc = block_sum<RT>(acc, red);
     if (threadIdx.x == 0)
-        out[blockIdx.x] = acc;
+        out[slot_map ? slot_map[blockIdx.x] : (int)blockIdx.x] = acc;
 }
 
-int launch_slot_sums(const double *partials, int n_slots, uint64_t n_tiles, double *out)
+int launch_slot_sums(const double *partials, int n_rows, uint64_t n_tiles, const int *slot_map, double *out)
 {
-    if (n_slots <= 0)
+    if (n_rows <= 0)
         return DMV_OK;
-    hipLaunchKernelGGL(k_slot_sums, dim3((uint32_t)n_slots), dim3(RT), 0, stream(), partials, n_tiles, out);
+    hipLaunchKernelGGL(k_slot_sums, dim3((uint32_t)n_rows), dim3(RT), 0, stream(), partials, n_tiles, slot_map, out);
     return check_launch("slot sums");
 }
 

@@ -6,14 +6,16 @@
 
 namespace dmv {
 
-// partials[slot][tile][wave]: one value per wave of every tile.
+// partials[ip][tile][wave] for the IP ops of this pass (ip = the op's index
+// among them): one value per wave of every tile, reduced by the host's
+// per-pass slot sums, so the buffer holds one pass's IP ops only.
 struct PartialSink {
     static constexpr bool kPerWave = true;
     double *partials;
-    uint64_t per_slot, base;  // per_slot = tiles x waves, base = tile x waves
-    __device__ void operator()(int slot, double v, uint32_t wave) const
+    uint64_t per_ip, base;  // per_ip = tiles x waves, base = tile x waves
+    __device__ void operator()(const DevOp *op, double v, uint32_t wave) const
     {
-        partials[(uint64_t)slot * per_slot + base + wave] = v;
+        partials[(uint64_t)op->ip_pos * per_ip + base + wave] = v;
     }
 };
 
@@ -46,13 +48,13 @@ int launch_tile_adjoint(int dtype, int rlog, int tlog, void *psi, void *lam, int
                         const DevPhase *phases, const DevOp *ops, uint32_t pass_idx, double *partials)
 {
     DMV_CHECK_ARG(n >= rlog + tlog && n - (rlog + tlog) <= 31, "tile shape does not fit %d qubits", n);
+    // psi and lambda are both held in registers: at most 8 (complex128)
+    // or 16 (complex64) amplitudes per thread and state
     if (dtype == DMV_C128) {
-        if (tlog == 8 && rlog == 4) return run_tile_adj<double, 4, 8>(psi, lam, n, L, passes, phases, ops, pass_idx, partials);
         if (tlog == 8 && rlog == 3) return run_tile_adj<double, 3, 8>(psi, lam, n, L, passes, phases, ops, pass_idx, partials);
         if (tlog == 8 && rlog == 2) return run_tile_adj<double, 2, 8>(psi, lam, n, L, passes, phases, ops, pass_idx, partials);
         if (tlog == 6 && rlog == 3) return run_tile_adj<double, 3, 6>(psi, lam, n, L, passes, phases, ops, pass_idx, partials);
     } else {
-        if (tlog == 8 && rlog == 5) return run_tile_adj<float, 5, 8>(psi, lam, n, L, passes, phases, ops, pass_idx, partials);
         if (tlog == 8 && rlog == 4) return run_tile_adj<float, 4, 8>(psi, lam, n, L, passes, phases, ops, pass_idx, partials);
         if (tlog == 8 && rlog == 3) return run_tile_adj<float, 3, 8>(psi, lam, n, L, passes, phases, ops, pass_idx, partials);
         if (tlog == 6 && rlog == 3) return run_tile_adj<float, 3, 6>(psi, lam, n, L, passes, phases, ops, pass_idx, partials);

@@ -12,8 +12,9 @@ namespace dmv {
 // state launcher below takes the state's layout L).
 int launch_tile(int dtype, int rlog, int tlog, bool nontemporal, void *state, int n, StateLayout L,
                 const DevPass *passes, const DevPhase *phases, const DevOp *ops, uint32_t pass_idx);
-// Reverse-sweep tile pass over psi and lambda; IP ops write one partial per
-// (slot, tile) into partials[slot * n_tiles + tile].
+// Reverse-sweep tile pass over psi and lambda; the IP ops of the pass write
+// one partial per (IP op, tile, wave) into
+// partials[(ip_pos * tiles + tile) * waves + wave].
 int launch_tile_adjoint(int dtype, int rlog, int tlog, void *psi, void *lam, int n, StateLayout L,
                         const DevPass *passes, const DevPhase *phases, const DevOp *ops,
                         uint32_t pass_idx, double *partials);
@@ -75,8 +76,9 @@ int launch_expect_zz(int dtype, const void *state, int n, StateLayout L, const d
 int launch_apply_obs(int dtype, const void *psi, void *lam, int n, StateLayout L, const double *a, int n_pairs,
                      const int *pi, const int *pj, const double *b, double *partials, double *out);
 int launch_inner(int dtype, const void *a, const void *b, int n, StateLayout L, double *partials, double *out2);
-// Sum each slot's partials: out[s] = sum_t partials[s * n_tiles + t].
-int launch_slot_sums(const double *partials, int n_slots, uint64_t n_tiles, double *out);
+// Row sums: out[slot_map ? slot_map[b] : b] = sum_t partials[b * n_tiles + t]
+// for b < n_rows.
+int launch_slot_sums(const double *partials, int n_rows, uint64_t n_tiles, const int *slot_map, double *out);
 int launch_sample(int dtype, const void *state, int n, StateLayout L, int64_t k, uint64_t seed, const double *u,
                   double *block_cum, double *partials, int64_t *out);
 

@@ -592,6 +592,20 @@ int build_plan(const std::vector<HostOp> &ops, const PlanParams &pp, Plan *out)
             Q |= uint64_t(1) << q;
         DMV_TRY(make_pass(Q, taken, pp, &plan));
     }
+    // number the IP ops of every pass: pass p's gradient partials are rows
+    // ip_begin[p] .. ip_begin[p+1] of the per-pass partial buffer
+    plan.ip_begin.assign(plan.passes.size() + 1, 0);
+    plan.ip_slots.clear();
+    for (size_t p = 0; p < plan.passes.size(); p++) {
+        plan.ip_begin[p] = (uint32_t)plan.ip_slots.size();
+        for (uint32_t ph = plan.passes[p].phase_begin; ph < plan.passes[p].phase_end; ph++)
+            for (uint32_t o = plan.phases[ph].op_begin; o < plan.phases[ph].op_end; o++)
+                if (op_is_ip(plan.ops[o].kind)) {
+                    plan.ops[o].ip_pos = (uint32_t)plan.ip_slots.size() - plan.ip_begin[p];
+                    plan.ip_slots.push_back(plan.ops[o].slot);
+                }
+    }
+    plan.ip_begin[plan.passes.size()] = (uint32_t)plan.ip_slots.size();
     DMV_TRY(validate_plan(plan, pp));
     *out = std::move(plan);
     return DMV_OK;

@@ -63,6 +63,10 @@ struct Plan {
     std::vector<DevOp> ops;      // tile-local ops, grouped by phase
     std::vector<int> op_pass;    // pass of each op
     int n_slots = 0;             // 1 + largest IP slot
+    // IP ops per pass: pass p holds ip_begin[p+1] - ip_begin[p] of them, whose
+    // slots (parameter indices) are ip_slots[ip_begin[p] ..] in ip_pos order
+    std::vector<uint32_t> ip_begin;
+    std::vector<int> ip_slots;
     // Every op in execution order with global qubits, including the X/CNOT
     // ops that only relabel registers and emit no device op.
     std::vector<HostOp> logical;

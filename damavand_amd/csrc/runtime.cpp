@@ -316,8 +316,10 @@ PlanParams tile_params(int n, int dtype, bool adjoint, StateLayout layout)
     int k = (int)tune_value(adjoint ? "adjoint_tile_qubits" : "tile_qubits", 0);
     int tlog = (int)tune_value("tile_threads_log2", 0);
     if (k == 0) {
+        // reverse sweeps keep psi and lambda in one register array: 2 x 8
+        // amplitudes per thread at k = 11
         if (dtype == DMV_C64)
-            k = adjoint ? 12 : 13;
+            k = adjoint ? 11 : 13;
         else
             k = adjoint ? 11 : 12;
     }
@@ -345,10 +347,12 @@ void free_plan(DevicePlan &dp)
     release(dp.d_passes);
     release(dp.d_phases);
     release(dp.d_ops);
+    release(dp.d_ip_slots);
     dp.d_passes = nullptr;
     dp.d_phases = nullptr;
     dp.d_ops = nullptr;
-    dp.cap_passes = dp.cap_phases = dp.cap_ops = 0;
+    dp.d_ip_slots = nullptr;
+    dp.cap_passes = dp.cap_phases = dp.cap_ops = dp.cap_ip = 0;
     dp.valid = false;
 }
 
@@ -393,6 +397,8 @@ int prepare_plan(DevicePlan &dp, uint64_t version, int n, int dtype, bool adjoin
         DMV_TRY(upload(dp.d_passes, dp.cap_passes, dp.plan.passes));
         DMV_TRY(upload(dp.d_phases, dp.cap_phases, dp.plan.phases));
         DMV_TRY(upload(dp.d_ops, dp.cap_ops, dp.plan.ops));
+        if (!dp.plan.ip_slots.empty())
+            DMV_TRY(upload(dp.d_ip_slots, dp.cap_ip, dp.plan.ip_slots));
     }
     dp.version = version;
     dp.epoch = tune_epoch();

@@ -56,7 +56,8 @@ struct DevicePlan {
     DevPass *d_passes = nullptr;
     DevPhase *d_phases = nullptr;
     DevOp *d_ops = nullptr;
-    size_t cap_passes = 0, cap_phases = 0, cap_ops = 0;
+    int *d_ip_slots = nullptr;    // plan.ip_slots (reverse-sweep plans)
+    size_t cap_passes = 0, cap_phases = 0, cap_ops = 0, cap_ip = 0;
     // hipGraph of the forward passes on one state buffer
     hipGraphExec_t graph = nullptr;
     const void *graph_state = nullptr;
@@ -70,7 +71,7 @@ struct dmv_circuit {
     std::vector<dmv::Gate> gates;
     uint64_t version = 0;
     dmv::DevicePlan fwd, adj;
-    double *d_partials = nullptr;   // adjoint per-(slot, tile) partials
+    double *d_partials = nullptr;   // adjoint partials of one pass: per (IP op, tile, wave)
     uint64_t cap_partials = 0;
     double *d_terms = nullptr;      // observable terms
     uint64_t cap_terms = 0;

@@ -493,7 +493,10 @@ __device__ __forceinline__ double ip_registers(const V (&p)[1 << RLOG], const V 
 
 // One tile of a reverse-sweep pass over psi and lambda.  For every IP op the
 // sum over each wave (Sink::kPerWave) or over the workgroup is handed to
-// sink(slot, value, wave) by the first lane of the wave / thread 0.
+// sink(op, value, wave) by the first lane of the wave / thread 0.  The ops
+// are applied in place, to psi and then to lambda: an out-of-place loop over
+// two states (four register arrays, or three with one run of ops per state
+// between IP ops) needed more registers or twice the code and ran slower.
 template <class R, int RLOG, int TLOG, class Sink>
 __device__ __forceinline__ void tile_body_adjoint(typename Cplx<R>::T *__restrict__ psi,
                                                   typename Cplx<R>::T *__restrict__ lam, const DevPass *P,
@@ -524,11 +527,11 @@ __device__ __forceinline__ void tile_body_adjoint(typename Cplx<R>::T *__restric
                     // one partial per wave: shuffles only, no workgroup barrier
                     const double w = wave_sum(ip_registers<V, RLOG>(p, l, op, tid));
                     if ((tid & 63) == 0)
-                        sink(op->slot, w, tid >> 6);
+                        sink(op, w, tid >> 6);
                 } else {
                     const double s = block_sum<NT>(ip_registers<V, RLOG>(p, l, op, tid), red);
                     if (tid == 0)
-                        sink(op->slot, s, 0);
+                        sink(op, s, 0);
                 }
                 continue;
             }
