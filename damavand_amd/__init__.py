@@ -8,8 +8,9 @@ runs in libdamavand-gpu.so (hand-written HIP kernels for gfx950); build it with
 """
 from ._native import DamavandError, NoDeviceError, OutOfDeviceMemory  # noqa: F401
 from .circuit import APPLY_METHODS, Circuit, StateVector, initialize_mpi  # noqa: F401
-from .batch import PackedBatch, batch_forward_adjoint, batch_forward_adjoint_packed, pack_circuits  # noqa: F401
+from .batch import (PackedBatch, batch_autotune, batch_forward_adjoint, batch_forward_adjoint_packed,  # noqa: F401
+                    pack_circuits)
 
 __all__ = ["Circuit", "StateVector", "initialize_mpi", "batch_forward_adjoint", "batch_forward_adjoint_packed",
-           "pack_circuits", "PackedBatch", "DamavandError",
+           "batch_autotune", "pack_circuits", "PackedBatch", "DamavandError",
            "NoDeviceError", "OutOfDeviceMemory", "APPLY_METHODS"]

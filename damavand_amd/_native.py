@@ -105,6 +105,12 @@ _SIGNATURES = {
     "dmv_tune_set": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int64]),
     "dmv_tune_get": (ctypes.c_int, [ctypes.c_char_p, _c_i64_p]),
     "dmv_autotune": (ctypes.c_int, [_vp, _vp, _c_int_p]),
+    "dmv_autotune_adjoint": (ctypes.c_int, [_vp, ctypes.c_int, _c_int_p]),
+    "dmv_tune_forget": (ctypes.c_int, []),
+    "dmv_batch_autotune": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _c_int_p, _c_int_p, _c_int_p, _c_int_p,
+                                          _c_int_p, _c_dbl_p, _c_int_p, _c_dbl_p, _c_int_p, _c_int_p]),
+    "dmv_circuit_tile_shape": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _c_int_p, _c_int_p, _c_int_p,
+                                              _c_int_p]),
     # compatibility layer (reference symbol names)
     "get_number_of_available_gpus": (ctypes.c_int, []),
     "get_memory_for_gpu": (ctypes.c_double, [ctypes.c_int]),

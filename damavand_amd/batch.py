@@ -1,6 +1,7 @@
 """Batched forward + adjoint of many small circuits in one persistent launch."""
 from __future__ import annotations
 
+import ctypes
 from typing import Sequence
 
 import numpy as np
@@ -108,6 +109,18 @@ def batch_forward_adjoint_packed(batch: PackedBatch, dtype="complex128"):
            N.iptr(batch.kinds), N.iptr(batch.ctrls), N.iptr(batch.tgts), N.dptr(batch.params),
            N.iptr(batch.coef_offset), N.dptr(batch.coef), N.iptr(batch.param_offset), N.dptr(energy), N.dptr(grad))
     return energy, grad[: batch.total_params]
+
+
+def batch_autotune(batch: PackedBatch, dtype="complex128") -> int:
+    """Times the batched kernel's launch shapes on this packed batch and keeps
+    the fastest for batches whose largest circuit has as many qubits (in this
+    process and in the file named by DMV_TUNE_CACHE).  Returns the chosen tile
+    qubits."""
+    k = ctypes.c_int(0)
+    N.call("dmv_batch_autotune", batch.count, _dtype_code(dtype), N.iptr(batch.n), N.iptr(batch.gate_offset),
+           N.iptr(batch.kinds), N.iptr(batch.ctrls), N.iptr(batch.tgts), N.dptr(batch.params),
+           N.iptr(batch.coef_offset), N.dptr(batch.coef), N.iptr(batch.param_offset), ctypes.byref(k))
+    return k.value
 
 
 def batch_forward_adjoint(circuits: Sequence, dtype="complex128"):

@@ -401,10 +401,26 @@ class Circuit:
                ctypes.byref(b))
         return p.value, ph.value, o.value, b.value
 
-    def autotune(self) -> int:
+    def autotune(self, adjoint: bool = False) -> int:
+        """Measures the fused-pass launch shapes of this circuit's forward (or,
+        with adjoint=True, of its gradient's reverse sweep) and keeps the
+        fastest for this circuit, for later circuits of the same size in this
+        process, and in the file named by DMV_TUNE_CACHE.  The candidates run
+        on scratch states; the circuit's state is left as it was.  Returns the
+        chosen tile qubits."""
         k = ctypes.c_int(0)
-        N.call("dmv_autotune", self._c.h, self.state.handle, ctypes.byref(k))
+        if adjoint:
+            N.call("dmv_autotune_adjoint", self._c.h, self.dtype, ctypes.byref(k))
+        else:
+            N.call("dmv_autotune", self._c.h, self.state.handle, ctypes.byref(k))
         return k.value
+
+    def tile_shape(self, adjoint: bool = False) -> dict:
+        """The launch shape the next forward (or reverse-sweep) plan uses."""
+        v = [ctypes.c_int(0) for _ in range(4)]
+        N.call("dmv_circuit_tile_shape", self._c.h, self.dtype, 1 if adjoint else 0, *[ctypes.byref(x) for x in v])
+        return {"tile_qubits": v[0].value, "threads_log2": v[1].value, "max_gen_per_pass": v[2].value,
+                "nontemporal": v[3].value}
 
     @property
     def handle(self):

@@ -1,5 +1,4 @@
-// Batched forward + adjoint of many small circuits in one persistent launch,
-// and the autotuner of the fused tile shape.
+// Batched forward + adjoint of many small circuits in one persistent launch.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -17,12 +16,36 @@
 
 namespace dmv {
 
-void tune_set(const char *name, int64_t v);
+// The batched kernel's tile shape (tile qubits k, log2 threads) and
+// workgroups per CU: from the caller's choice `pin`, else the tunables
+// batch_tile_qubits / batch_threads_log2 / batch_wgs_per_cu, else the tuning
+// table for (qubits of the largest circuit, dtype), else k = 11 on 256
+// threads, two workgroups per CU.
+static void batch_shape(int nmax, int dtype, const TuneChoice *pin, int *k, int *tlog, int *wgs)
+{
+    *k = 11;
+    *tlog = 8;
+    *wgs = 2;
+    TuneChoice t;
+    if (tune_lookup(tune_key(nmax, dtype, "batch"), &t) && t.set()) {
+        *k = t.k;
+        *tlog = t.tlog > 0 ? t.tlog : *tlog;
+        *wgs = t.wgs > 0 ? t.wgs : *wgs;
+    }
+    *k = (int)tune_value("batch_tile_qubits", *k);
+    *tlog = (int)tune_value("batch_threads_log2", *tlog);
+    *wgs = (int)tune_value("batch_wgs_per_cu", *wgs);
+    if (pin && pin->set()) {
+        *k = pin->k;
+        *tlog = pin->tlog > 0 ? pin->tlog : *tlog;
+        *wgs = pin->wgs > 0 ? pin->wgs : *wgs;
+    }
+}
 
 int batch_forward_adjoint(int count, int dtype, const int *n, const int *gate_offset, const int *gate_kind,
                           const int *gate_control, const int *gate_target, const double *gate_param,
                           const int *coef_offset, const double *coef, const int *param_offset, double *energy,
-                          double *grad)
+                          double *grad, const TuneChoice *pin, double *kernel_ms)
 {
     DMV_CHECK_ARG(count >= 1, "empty batch");
     DMV_CHECK_ARG(dtype == DMV_C128 || dtype == DMV_C64, "bad dtype");
@@ -38,9 +61,15 @@ int batch_forward_adjoint(int count, int dtype, const int *n, const int *gate_of
         DMV_CHECK_ARG(gate_offset[i + 1] >= gate_offset[i], "gate_offset decreases at circuit %d", i);
         DMV_CHECK_ARG(coef_offset[i] >= 0, "coef_offset[%d] is negative", i);
     }
-    int rlog, tlog;
-    batch_tile_shape(dtype, &rlog, &tlog);
-    const int K = rlog + tlog;
+    int nmax_all = 0;
+    for (int i = 0; i < count; i++)
+        nmax_all = std::max(nmax_all, n[i]);
+    int K, tlog, per_cu;
+    batch_shape(nmax_all, dtype, pin, &K, &tlog, &per_cu);
+    const int rlog = K - tlog;
+    DMV_CHECK_ARG(batch_kernel_exists(dtype, rlog, tlog), "no batched kernel for %d tile qubits on %d threads", K,
+                  1 << tlog);
+    DMV_CHECK_ARG(per_cu >= 1 && per_cu <= 16, "batch_wgs_per_cu must be in [1, 16], got %d", per_cu);
     const int small_max = K - 1;
 
     // Per-circuit plans are independent: build them on host threads, then
@@ -190,8 +219,7 @@ int batch_forward_adjoint(int count, int dtype, const int *n, const int *gate_of
 
     // host work and argument checks are done: only now touch the device
     DMV_TRY(ensure_device());
-    const int per_cu = (int)tune_value("batch_wgs_per_cu", 2);
-    uint32_t grid = (uint32_t)std::min<int64_t>(count, (int64_t)device_cus() * std::max(1, per_cu));
+    uint32_t grid = (uint32_t)std::min<int64_t>(count, (int64_t)device_cus() * per_cu);
     const uint64_t slot_amps = uint64_t(1) << std::max(nmax, K);
     const uint64_t scratch_bytes = (uint64_t)grid * 2 * slot_amps * amp_bytes(dtype);
 
@@ -238,8 +266,29 @@ int batch_forward_adjoint(int count, int dtype, const int *n, const int *gate_of
     L.energy = (double *)b_energy.p;
     L.grad = (double *)b_grad.p;
     L.grid = grid;
+    L.rlog = rlog;
+    L.tlog = tlog;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (kernel_ms && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) {
+        if (e0)
+            (void)hipEventDestroy(e0);
+        set_error("hipEventCreate failed");
+        return DMV_ERR_HIP;
+    }
+    if (kernel_ms)
+        (void)hipEventRecord(e0, stream());
     prof_begin(DMV_KC_BATCH);
-    DMV_TRY(launch_batch(dtype, L));
+    int rc = launch_batch(dtype, L);
+    if (kernel_ms) {
+        (void)hipEventRecord(e1, stream());
+        float ms = 0.f;
+        if (rc == DMV_OK && hipEventSynchronize(e1) == hipSuccess)
+            (void)hipEventElapsedTime(&ms, e0, e1);
+        *kernel_ms = ms;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+    DMV_TRY(rc);
     double bytes = 0.0;
     for (int i = 0; i < count; i++) {
         const double sb = std::ldexp((double)amp_bytes(dtype), n[i]);
@@ -253,77 +302,6 @@ int batch_forward_adjoint(int count, int dtype, const int *n, const int *gate_of
     DMV_TRY(copy_d2h(energy, b_energy.p, (uint64_t)count * sizeof(double)));
     if (total_params > 0)
         DMV_TRY(copy_d2h(grad, b_grad.p, (uint64_t)total_params * sizeof(double)));
-    return DMV_OK;
-}
-
-// ---------------------------------------------------------------- autotune --
-static std::string cache_key(int n, int dtype) { return "n" + std::to_string(n) + (dtype == DMV_C64 ? "c64" : "c128"); }
-
-int autotune(dmv_circuit *c, dmv_state *s, int *chosen)
-{
-    DMV_CHECK_ARG(c->n == s->n, "circuit/state size mismatch");
-    DMV_TRY(ensure_device());
-    const int small_cap = s->dtype == DMV_C64 ? kSmallMaxQubitsC64 : kSmallMaxQubitsC128;
-    if (s->n <= small_cap) {
-        if (chosen)
-            *chosen = s->n;
-        return DMV_OK;
-    }
-    std::vector<int> cands = s->dtype == DMV_C64 ? std::vector<int>{11, 12, 13} : std::vector<int>{10, 11, 12};
-    std::vector<int> gens = {8, 16, 32};
-    double best = 1e300;
-    int best_k = 0, best_gen = 24, best_nt = 0;
-    hipEvent_t e0, e1;
-    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
-        set_error("hipEventCreate failed");
-        return DMV_ERR_HIP;
-    }
-    for (int k : cands) {
-        if (k + 6 > s->n)
-            continue;
-        for (int gen : gens)
-            for (int nt = 0; nt < 2; nt++) {
-                tune_set("tile_qubits", k);
-                tune_set("max_gen_per_pass", gen);
-                tune_set("nontemporal", nt);
-                int rc = circuit_forward(c, s, DMV_FORWARD_FUSED);  // builds the plan, warms up
-                if (rc != DMV_OK)
-                    return rc;
-                (void)hipEventRecord(e0, stream());
-                rc = circuit_forward(c, s, DMV_FORWARD_FUSED);
-                (void)hipEventRecord(e1, stream());
-                (void)hipEventSynchronize(e1);
-                if (rc != DMV_OK)
-                    return rc;
-                float ms = 0.f;
-                (void)hipEventElapsedTime(&ms, e0, e1);
-                if (ms < best) {
-                    best = ms;
-                    best_k = k;
-                    best_gen = gen;
-                    best_nt = nt;
-                }
-            }
-    }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    if (best_k == 0) {
-        if (chosen)
-            *chosen = 0;
-        return DMV_OK;
-    }
-    tune_set("tile_qubits", best_k);
-    tune_set("max_gen_per_pass", best_gen);
-    tune_set("nontemporal", best_nt);
-    if (const char *path = getenv("DMV_TUNE_CACHE")) {
-        if (FILE *f = fopen(path, "a")) {
-            fprintf(f, "{\"key\": \"%s\", \"tile_qubits\": %d, \"max_gen_per_pass\": %d, \"nontemporal\": %d, \"ms\": %.6f}\n",
-                    cache_key(s->n, s->dtype).c_str(), best_k, best_gen, best_nt, best);
-            fclose(f);
-        }
-    }
-    if (chosen)
-        *chosen = best_k;
     return DMV_OK;
 }
 

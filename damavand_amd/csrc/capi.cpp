@@ -18,8 +18,13 @@ void tune_set(const char *name, int64_t v);
 int batch_forward_adjoint(int count, int dtype, const int *n, const int *gate_offset, const int *gate_kind,
                           const int *gate_control, const int *gate_target, const double *gate_param,
                           const int *coef_offset, const double *coef, const int *param_offset, double *energy,
-                          double *grad);
-int autotune(dmv_circuit *c, dmv_state *s, int *chosen);
+                          double *grad, const TuneChoice *pin, double *kernel_ms);
+int autotune_forward(dmv_circuit *c, dmv_state *s, int *chosen);
+int autotune_adjoint(dmv_circuit *c, int dtype, int *chosen);
+void tune_table_reset();
+int autotune_batch(int count, int dtype, const int *n, const int *gate_offset, const int *gate_kind,
+                   const int *gate_control, const int *gate_target, const double *gate_param, const int *coef_offset,
+                   const double *coef, const int *param_offset, int *chosen);
 }  // namespace dmv
 
 namespace dmv {
@@ -49,6 +54,126 @@ using namespace dmv;
 
 #define CHECK_STATE(s) DMV_CHECK_ARG((s) != nullptr && (s)->amp != nullptr, "null state")
 #define CHECK_CIRCUIT(c) DMV_CHECK_ARG((c) != nullptr, "null circuit")
+
+// Uploads a Z/ZZ observable into a device buffer laid out as
+// a[n] (double) | b[n_pairs] (double) | pi[n_pairs] (int) | pj[n_pairs] (int).
+static int upload_terms(double **buf, uint64_t *cap, int n, const double *a, int n_pairs, const int *pi,
+                        const int *pj, const double *b, const double **da, const double **db, const int **dpi,
+                        const int **dpj)
+{
+    DMV_CHECK_ARG(n_pairs >= 0 && n_pairs <= 128, "at most 128 ZZ terms");
+    DMV_CHECK_ARG(n_pairs == 0 || (pi && pj && b), "null ZZ term arrays");
+    for (int t = 0; t < n_pairs; t++)
+        DMV_CHECK_ARG(pi[t] >= 0 && pi[t] < n && pj[t] >= 0 && pj[t] < n, "ZZ term qubit out of range");
+    const uint64_t need = (uint64_t)n + (uint64_t)n_pairs * 2 + 8;
+    if (need > *cap) {
+        release(*buf);
+        *buf = nullptr;
+        DMV_TRY(alloc((void **)buf, need * sizeof(double)));
+        *cap = need;
+    }
+    std::vector<double> host(need, 0.0);
+    for (int q = 0; q < n; q++)
+        host[q] = a ? a[q] : 0.0;
+    for (int t = 0; t < n_pairs; t++)
+        host[n + t] = b[t];
+    int *ip = reinterpret_cast<int *>(host.data() + n + n_pairs);
+    for (int t = 0; t < n_pairs; t++) {
+        ip[t] = pi[t];
+        ip[n_pairs + t] = pj[t];
+    }
+    DMV_TRY(copy_h2d(*buf, host.data(), need * sizeof(double)));
+    *da = *buf;
+    *db = *buf + n;
+    *dpi = reinterpret_cast<const int *>(*buf + n + n_pairs);
+    *dpj = *dpi + n_pairs;
+    return DMV_OK;
+}
+
+int dmv::circuit_adjoint(dmv_circuit *c, dmv_state *psi, dmv_state *lam, const double *a, int n_pairs, const int *pi,
+                         const int *pj, const double *b, double *energy, double *grad, DevicePlan &dp,
+                         const TuneChoice *pin)
+{
+    CHECK_CIRCUIT(c);
+    CHECK_STATE(psi);
+    CHECK_STATE(lam);
+    DMV_CHECK_ARG(psi != lam, "psi and lambda must be different states");
+    DMV_CHECK_ARG(psi->n == c->n && lam->n == c->n && psi->dtype == lam->dtype, "state/circuit size mismatch");
+    DMV_CHECK_ARG(same_layout(psi->layout, lam->layout), "psi and lambda differ in element order");
+    DMV_TRY(ensure_device());
+    int np = 0;
+    dmv_circuit_num_parameters(c, &np);
+    // psi = U|0>
+    DMV_TRY(state_reset(psi, 0, false));
+    DMV_TRY(circuit_forward(c, psi, DMV_FORWARD_FUSED));
+    // lambda = H psi, E = <psi|H|psi>
+    const double *da, *db;
+    const int *dpi, *dpj;
+    DMV_TRY(upload_terms(&c->d_terms, &c->cap_terms, c->n, a, n_pairs, pi, pj, b, &da, &db, &dpi, &dpj));
+    prof_begin(DMV_KC_REDUCE);
+    DMV_TRY(launch_apply_obs(psi->dtype, psi->amp, lam->amp, c->n, psi->layout, da, n_pairs, dpi, dpj, db, psi->ws, psi->scalars));
+    prof_end(DMV_KC_REDUCE, 3.0 * (double)psi->bytes);
+    // reverse sweep
+    DMV_TRY(prepare_plan(dp, c->version, c->n, psi->dtype, true, adjoint_ops(c->gates), psi->layout, pin));
+    if (dp.plan.n_slots != np) {
+        set_error("internal: adjoint plan has %d slots for %d parameters", dp.plan.n_slots, np);
+        return DMV_ERR_STATE;
+    }
+    // gradient partials: per (IP op, tile, wave) on the tile path, per slot
+    // on the whole-state path.  The tile path keeps the rows of as many
+    // consecutive passes as fit in kPartialBudget bytes (at least one pass)
+    // and sums them into grad with one launch whenever the next pass would
+    // not fit, so the buffer stays bounded at any state size.
+    constexpr uint64_t kPartialBudget = uint64_t(256) << 20;
+    uint64_t per_ip = 1, rows = (uint64_t)(np > 0 ? np : 1);
+    if (!dp.small) {
+        per_ip = (uint64_t(1) << (c->n - dp.pp.k)) * ((uint64_t(1) << dp.pp.tlog) / 64);
+        uint64_t widest = 1;
+        for (size_t p = 0; p < dp.plan.passes.size(); p++)
+            widest = std::max<uint64_t>(widest, dp.plan.ip_begin[p + 1] - dp.plan.ip_begin[p]);
+        rows = std::max(widest, std::min<uint64_t>(std::max<uint64_t>(1, dp.plan.ip_slots.size()),
+                                                   kPartialBudget / (per_ip * sizeof(double))));
+    }
+    if (rows * per_ip > c->cap_partials) {
+        release(c->d_partials);
+        c->d_partials = nullptr;
+        DMV_TRY(alloc((void **)&c->d_partials, rows * per_ip * sizeof(double)));
+        c->cap_partials = rows * per_ip;
+    }
+    if ((uint64_t)np + 1 > c->cap_grad) {
+        release(c->d_grad);
+        c->d_grad = nullptr;
+        DMV_TRY(alloc((void **)&c->d_grad, ((uint64_t)np + 1) * sizeof(double)));
+        c->cap_grad = (uint64_t)np + 1;
+    }
+    if (dp.small) {
+        prof_begin(DMV_KC_ADJOINT);
+        DMV_TRY(launch_small_adjoint(psi->dtype, psi->amp, lam->amp, c->n, dp.d_ops, (uint32_t)dp.small_ops.size(),
+                                     c->d_partials));
+        prof_end(DMV_KC_ADJOINT, 4.0 * (double)psi->bytes);
+        DMV_TRY(launch_slot_sums(c->d_partials, np, 1, nullptr, c->d_grad));
+    } else {
+        const uint32_t n_pass = (uint32_t)dp.plan.passes.size();
+        uint32_t pending = 0;  // first IP row not yet summed
+        for (uint32_t p = 0; p < n_pass; p++) {
+            const uint64_t row0 = dp.plan.ip_begin[p] - pending;
+            prof_begin(DMV_KC_ADJOINT);
+            DMV_TRY(launch_tile_adjoint(psi->dtype, dp.pp.rlog, dp.pp.tlog, psi->amp, lam->amp, c->n, psi->layout, dp.d_passes,
+                                        dp.d_phases, dp.d_ops, p, c->d_partials + row0 * per_ip));
+            prof_end(DMV_KC_ADJOINT, 4.0 * (double)psi->bytes);
+            const uint32_t done = dp.plan.ip_begin[p + 1];
+            if (p + 1 == n_pass || dp.plan.ip_begin[p + 2] - pending > rows) {
+                DMV_TRY(launch_slot_sums(c->d_partials, (int)(done - pending), per_ip, dp.d_ip_slots + pending, c->d_grad));
+                pending = done;
+            }
+        }
+    }
+    if (energy)
+        DMV_TRY(copy_d2h(energy, psi->scalars, sizeof(double)));
+    if (grad && np > 0)
+        DMV_TRY(copy_d2h(grad, c->d_grad, sizeof(double) * np));
+    return DMV_OK;
+}
 
 extern "C" {
 
@@ -294,41 +419,6 @@ int dmv_expect_z(dmv_state *s, double *out)
     return copy_d2h(out, s->scalars, sizeof(double) * s->n);
 }
 
-// Uploads a Z/ZZ observable into a device buffer laid out as
-// a[n] (double) | b[n_pairs] (double) | pi[n_pairs] (int) | pj[n_pairs] (int).
-static int upload_terms(double **buf, uint64_t *cap, int n, const double *a, int n_pairs, const int *pi,
-                        const int *pj, const double *b, const double **da, const double **db, const int **dpi,
-                        const int **dpj)
-{
-    DMV_CHECK_ARG(n_pairs >= 0 && n_pairs <= 128, "at most 128 ZZ terms");
-    DMV_CHECK_ARG(n_pairs == 0 || (pi && pj && b), "null ZZ term arrays");
-    for (int t = 0; t < n_pairs; t++)
-        DMV_CHECK_ARG(pi[t] >= 0 && pi[t] < n && pj[t] >= 0 && pj[t] < n, "ZZ term qubit out of range");
-    const uint64_t need = (uint64_t)n + (uint64_t)n_pairs * 2 + 8;
-    if (need > *cap) {
-        release(*buf);
-        *buf = nullptr;
-        DMV_TRY(alloc((void **)buf, need * sizeof(double)));
-        *cap = need;
-    }
-    std::vector<double> host(need, 0.0);
-    for (int q = 0; q < n; q++)
-        host[q] = a ? a[q] : 0.0;
-    for (int t = 0; t < n_pairs; t++)
-        host[n + t] = b[t];
-    int *ip = reinterpret_cast<int *>(host.data() + n + n_pairs);
-    for (int t = 0; t < n_pairs; t++) {
-        ip[t] = pi[t];
-        ip[n_pairs + t] = pj[t];
-    }
-    DMV_TRY(copy_h2d(*buf, host.data(), need * sizeof(double)));
-    *da = *buf;
-    *db = *buf + n;
-    *dpi = reinterpret_cast<const int *>(*buf + n + n_pairs);
-    *dpj = *dpi + n_pairs;
-    return DMV_OK;
-}
-
 int dmv_expect_zz(dmv_state *s, const double *a, int n_pairs, const int *pi, const int *pj, const double *b,
                   double *out)
 {
@@ -531,6 +621,8 @@ int dmv_circuit_clear(dmv_circuit *c)
     CHECK_CIRCUIT(c);
     c->gates.clear();
     c->version++;
+    c->tune_fwd = TuneChoice();  // tuned for the gates just removed
+    c->tune_adj = TuneChoice();
     return DMV_OK;
 }
 
@@ -544,86 +636,7 @@ int dmv_circuit_forward(dmv_circuit *c, dmv_state *s, int mode)
 int dmv_circuit_adjoint(dmv_circuit *c, dmv_state *psi, dmv_state *lam, const double *a, int n_pairs, const int *pi,
                         const int *pj, const double *b, double *energy, double *grad)
 {
-    CHECK_CIRCUIT(c);
-    CHECK_STATE(psi);
-    CHECK_STATE(lam);
-    DMV_CHECK_ARG(psi != lam, "psi and lambda must be different states");
-    DMV_CHECK_ARG(psi->n == c->n && lam->n == c->n && psi->dtype == lam->dtype, "state/circuit size mismatch");
-    DMV_CHECK_ARG(same_layout(psi->layout, lam->layout), "psi and lambda differ in element order");
-    DMV_TRY(ensure_device());
-    int np = 0;
-    dmv_circuit_num_parameters(c, &np);
-    // psi = U|0>
-    DMV_TRY(state_reset(psi, 0, false));
-    DMV_TRY(circuit_forward(c, psi, DMV_FORWARD_FUSED));
-    // lambda = H psi, E = <psi|H|psi>
-    const double *da, *db;
-    const int *dpi, *dpj;
-    DMV_TRY(upload_terms(&c->d_terms, &c->cap_terms, c->n, a, n_pairs, pi, pj, b, &da, &db, &dpi, &dpj));
-    prof_begin(DMV_KC_REDUCE);
-    DMV_TRY(launch_apply_obs(psi->dtype, psi->amp, lam->amp, c->n, psi->layout, da, n_pairs, dpi, dpj, db, psi->ws, psi->scalars));
-    prof_end(DMV_KC_REDUCE, 3.0 * (double)psi->bytes);
-    // reverse sweep
-    DMV_TRY(prepare_plan(c->adj, c->version, c->n, psi->dtype, true, adjoint_ops(c->gates), psi->layout));
-    DevicePlan &dp = c->adj;
-    if (dp.plan.n_slots != np) {
-        set_error("internal: adjoint plan has %d slots for %d parameters", dp.plan.n_slots, np);
-        return DMV_ERR_STATE;
-    }
-    // gradient partials: per (IP op, tile, wave) on the tile path, per slot
-    // on the whole-state path.  The tile path keeps the rows of as many
-    // consecutive passes as fit in kPartialBudget bytes (at least one pass)
-    // and sums them into grad with one launch whenever the next pass would
-    // not fit, so the buffer stays bounded at any state size.
-    constexpr uint64_t kPartialBudget = uint64_t(256) << 20;
-    uint64_t per_ip = 1, rows = (uint64_t)(np > 0 ? np : 1);
-    if (!dp.small) {
-        per_ip = (uint64_t(1) << (c->n - dp.pp.k)) * ((uint64_t(1) << dp.pp.tlog) / 64);
-        uint64_t widest = 1;
-        for (size_t p = 0; p < dp.plan.passes.size(); p++)
-            widest = std::max<uint64_t>(widest, dp.plan.ip_begin[p + 1] - dp.plan.ip_begin[p]);
-        rows = std::max(widest, std::min<uint64_t>(std::max<uint64_t>(1, dp.plan.ip_slots.size()),
-                                                   kPartialBudget / (per_ip * sizeof(double))));
-    }
-    if (rows * per_ip > c->cap_partials) {
-        release(c->d_partials);
-        c->d_partials = nullptr;
-        DMV_TRY(alloc((void **)&c->d_partials, rows * per_ip * sizeof(double)));
-        c->cap_partials = rows * per_ip;
-    }
-    if ((uint64_t)np + 1 > c->cap_grad) {
-        release(c->d_grad);
-        c->d_grad = nullptr;
-        DMV_TRY(alloc((void **)&c->d_grad, ((uint64_t)np + 1) * sizeof(double)));
-        c->cap_grad = (uint64_t)np + 1;
-    }
-    if (dp.small) {
-        prof_begin(DMV_KC_ADJOINT);
-        DMV_TRY(launch_small_adjoint(psi->dtype, psi->amp, lam->amp, c->n, dp.d_ops, (uint32_t)dp.small_ops.size(),
-                                     c->d_partials));
-        prof_end(DMV_KC_ADJOINT, 4.0 * (double)psi->bytes);
-        DMV_TRY(launch_slot_sums(c->d_partials, np, 1, nullptr, c->d_grad));
-    } else {
-        const uint32_t n_pass = (uint32_t)dp.plan.passes.size();
-        uint32_t pending = 0;  // first IP row not yet summed
-        for (uint32_t p = 0; p < n_pass; p++) {
-            const uint64_t row0 = dp.plan.ip_begin[p] - pending;
-            prof_begin(DMV_KC_ADJOINT);
-            DMV_TRY(launch_tile_adjoint(psi->dtype, dp.pp.rlog, dp.pp.tlog, psi->amp, lam->amp, c->n, psi->layout, dp.d_passes,
-                                        dp.d_phases, dp.d_ops, p, c->d_partials + row0 * per_ip));
-            prof_end(DMV_KC_ADJOINT, 4.0 * (double)psi->bytes);
-            const uint32_t done = dp.plan.ip_begin[p + 1];
-            if (p + 1 == n_pass || dp.plan.ip_begin[p + 2] - pending > rows) {
-                DMV_TRY(launch_slot_sums(c->d_partials, (int)(done - pending), per_ip, dp.d_ip_slots + pending, c->d_grad));
-                pending = done;
-            }
-        }
-    }
-    if (energy)
-        DMV_TRY(copy_d2h(energy, psi->scalars, sizeof(double)));
-    if (grad && np > 0)
-        DMV_TRY(copy_d2h(grad, c->d_grad, sizeof(double) * np));
-    return DMV_OK;
+    return circuit_adjoint(c, psi, lam, a, n_pairs, pi, pj, b, energy, grad, c->adj, &c->tune_adj);
 }
 
 int dmv_circuit_plan_stats(dmv_circuit *c, int dtype, int *passes, int *phases, int *ops, double *hbm_bytes)
@@ -640,7 +653,7 @@ int dmv_circuit_plan_stats(dmv_circuit *c, int dtype, int *passes, int *phases, 
         return DMV_OK;
     }
     Plan plan;
-    DMV_TRY(build_plan(merged, tile_params(c->n, dtype, false, default_layout(c->n, dtype)), &plan));
+    DMV_TRY(build_plan(merged, tile_params(c->n, dtype, false, default_layout(c->n, dtype), &c->tune_fwd), &plan));
     if (passes) *passes = (int)plan.passes.size();
     if (phases) *phases = (int)plan.phases.size();
     if (ops) *ops = (int)plan.ops.size();
@@ -652,7 +665,7 @@ int dmv_circuit_plan_ops(dmv_circuit *c, int dtype, int tile_qubits, int *kind, 
                          double *m, int *pass, int capacity, int *count)
 {
     CHECK_CIRCUIT(c);
-    PlanParams pp = tile_params(c->n, dtype, false, default_layout(c->n, dtype));
+    PlanParams pp = tile_params(c->n, dtype, false, default_layout(c->n, dtype), &c->tune_fwd);
     if (tile_qubits > 0) {
         pp.k = tile_qubits;
         pp.tlog = tile_qubits >= 10 ? 8 : 6;
@@ -717,7 +730,8 @@ int dmv_circuit_plan_json(dmv_circuit *c, int dtype, int adjoint, int tile_qubit
                           int max_gen_per_pass, char *buffer, uint64_t capacity, uint64_t *length)
 {
     CHECK_CIRCUIT(c);
-    PlanParams pp = tile_params(c->n, dtype, adjoint != 0, default_layout(c->n, dtype));
+    PlanParams pp = tile_params(c->n, dtype, adjoint != 0, default_layout(c->n, dtype),
+                                adjoint ? &c->tune_adj : &c->tune_fwd);
     if (tile_qubits > 0) {
         pp.k = tile_qubits;
         pp.tlog = threads_log2 > 0 ? threads_log2 : (tile_qubits >= 10 ? 8 : 6);
@@ -797,7 +811,15 @@ int dmv_batch_forward_adjoint(int count, int dtype, const int *n, const int *gat
                               double *grad)
 {
     return batch_forward_adjoint(count, dtype, n, gate_offset, gate_kind, gate_control, gate_target, gate_param,
-                                 coef_offset, coef, param_offset, energy, grad);
+                                 coef_offset, coef, param_offset, energy, grad, nullptr, nullptr);
+}
+
+int dmv_batch_autotune(int count, int dtype, const int *n, const int *gate_offset, const int *gate_kind,
+                       const int *gate_control, const int *gate_target, const double *gate_param,
+                       const int *coef_offset, const double *coef, const int *param_offset, int *chosen_tile_qubits)
+{
+    return autotune_batch(count, dtype, n, gate_offset, gate_kind, gate_control, gate_target, gate_param, coef_offset,
+                          coef, param_offset, chosen_tile_qubits);
 }
 
 // --------------------------------------------------------------- profiling --
@@ -863,7 +885,37 @@ int dmv_autotune(dmv_circuit *c, dmv_state *s, int *chosen_tile_qubits)
 {
     CHECK_CIRCUIT(c);
     CHECK_STATE(s);
-    return autotune(c, s, chosen_tile_qubits);
+    return autotune_forward(c, s, chosen_tile_qubits);
+}
+
+int dmv_tune_forget(void)
+{
+    tune_table_reset();
+    return DMV_OK;
+}
+
+int dmv_autotune_adjoint(dmv_circuit *c, int dtype, int *chosen_tile_qubits)
+{
+    CHECK_CIRCUIT(c);
+    return autotune_adjoint(c, dtype, chosen_tile_qubits);
+}
+
+int dmv_circuit_tile_shape(dmv_circuit *c, int dtype, int adjoint, int *tile_qubits, int *threads_log2,
+                           int *max_gen_per_pass, int *nontemporal)
+{
+    CHECK_CIRCUIT(c);
+    DMV_CHECK_ARG(dtype == DMV_C128 || dtype == DMV_C64, "dtype must be DMV_C128 or DMV_C64");
+    const PlanParams pp = tile_params(c->n, dtype, adjoint != 0, default_layout(c->n, dtype),
+                                      adjoint ? &c->tune_adj : &c->tune_fwd);
+    if (tile_qubits)
+        *tile_qubits = pp.k;
+    if (threads_log2)
+        *threads_log2 = pp.tlog;
+    if (max_gen_per_pass)
+        *max_gen_per_pass = pp.max_gen_per_pass;
+    if (nontemporal)
+        *nontemporal = pp.nontemporal;
+    return DMV_OK;
 }
 
 }  // extern "C"

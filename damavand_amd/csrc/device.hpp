@@ -105,7 +105,6 @@ hipStream_t stream();
 void prof_begin(int kernel_class);
 void prof_end(int kernel_class, double bytes);
 int check_launch(const char *what);
-bool tune_nontemporal();
 int64_t tune_value(const char *name, int64_t dflt);
 int gate_pairs_per_thread();
 bool gate_nontemporal(uint32_t target, int dtype);

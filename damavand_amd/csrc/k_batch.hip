@@ -157,22 +157,32 @@ static int run_batch(const BatchLaunch &L)
     return check_launch("batch kernel");
 }
 
-// 256 threads and 8 amplitudes per thread and state (k = 11): the shape of
-// the reverse-sweep passes, which hold psi and lambda in registers together.
-int batch_tile_shape(int dtype, int *rlog, int *tlog)
+// Tile shapes (log2 amplitudes per thread and state, log2 threads) of the
+// batched kernel: 8 amplitudes per thread and state at most, since the
+// reverse sweep holds psi and lambda in registers together.
+#define DMV_BATCH_SHAPES(X) X(3, 8) X(2, 8) X(3, 7) X(3, 9)
+
+bool batch_kernel_exists(int dtype, int rlog, int tlog)
 {
     (void)dtype;
-    *tlog = 8;
-    *rlog = 3;
-    return DMV_OK;
+#define DMV_BATCH_HAS(R, T)                                                                        \
+    if (rlog == R && tlog == T)                                                                    \
+        return true;
+    DMV_BATCH_SHAPES(DMV_BATCH_HAS)
+#undef DMV_BATCH_HAS
+    return false;
 }
 
 int launch_batch(int dtype, const BatchLaunch &L)
 {
     DMV_CHECK_ARG(L.grid >= 1 && L.count >= 1, "empty batch");
-    if (dtype == DMV_C64)
-        return run_batch<float, 3, 8>(L);
-    return run_batch<double, 3, 8>(L);
+#define DMV_BATCH_RUN(R, T)                                                                        \
+    if (L.rlog == R && L.tlog == T)                                                                \
+        return dtype == DMV_C64 ? run_batch<float, R, T>(L) : run_batch<double, R, T>(L);
+    DMV_BATCH_SHAPES(DMV_BATCH_RUN)
+#undef DMV_BATCH_RUN
+    set_error("no batched kernel for rlog %d tlog %d", L.rlog, L.tlog);
+    return DMV_ERR_ARG;
 }
 
 }  // namespace dmv

@@ -49,8 +49,10 @@ struct BatchLaunch {
     double *energy;
     double *grad;
     uint32_t grid;
+    int rlog, tlog;       // tile shape: 2^rlog amplitudes per thread, 2^tlog threads
 };
-int batch_tile_shape(int dtype, int *rlog, int *tlog);
+// Whether the batched kernel is built for this tile shape.
+bool batch_kernel_exists(int dtype, int rlog, int tlog);
 int launch_batch(int dtype, const BatchLaunch &L);
 
 // One op as its own pass over the amplitudes it changes (global qubits).

@@ -54,6 +54,7 @@ struct PlanParams {
     int low_bits;   // lowest qubits kept in lanes for coalescing
     int swz_bits;   // LDS swizzle group (3 for 16-byte, 4 for 8-byte amplitudes)
     int max_gen_per_pass;  // compute budget of a pass, in general 2x2 ops
+    int nontemporal = 0;   // forward passes: streaming (nontemporal) loads and stores
     StateLayout layout = kPlainLayout;  // element map of the states the plan runs on
 };
 

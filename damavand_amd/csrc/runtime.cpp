@@ -268,7 +268,6 @@ int64_t tune_value(const char *name, int64_t dflt)
     return dflt;
 }
 
-bool tune_nontemporal() { return tune_value("nontemporal", 0) != 0; }
 
 // Layout of a new state.  The span index is folded on states of 2^15 spans
 // and more (32 qubits in complex128): there, tiles over the highest qubits
@@ -301,7 +300,7 @@ bool gate_nontemporal(uint32_t target, int dtype)
 }
 
 // ------------------------------------------------------------------- plans --
-PlanParams tile_params(int n, int dtype, bool adjoint, StateLayout layout)
+PlanParams tile_params(int n, int dtype, bool adjoint, StateLayout layout, const TuneChoice *pin)
 {
     PlanParams pp;
     pp.n = n;
@@ -312,21 +311,37 @@ PlanParams tile_params(int n, int dtype, bool adjoint, StateLayout layout)
     if (runs > pp.low_bits && runs <= 6)
         pp.low_bits = (int)runs;
     pp.swz_bits = dtype == DMV_C64 ? 4 : 3;
-    pp.max_gen_per_pass = (int)tune_value("max_gen_per_pass", 8);
-    int k = (int)tune_value(adjoint ? "adjoint_tile_qubits" : "tile_qubits", 0);
-    int tlog = (int)tune_value("tile_threads_log2", 0);
-    if (k == 0) {
-        // reverse sweeps keep psi and lambda in one register array: 2 x 8
-        // amplitudes per thread at k = 11
-        if (dtype == DMV_C64)
-            k = adjoint ? 11 : 13;
-        else
-            k = adjoint ? 11 : 12;
+    // defaults: 256 threads; k = 12 (complex128) / 13 (complex64) forward,
+    // 11 for reverse sweeps, which hold psi and lambda in registers together
+    int k = dtype == DMV_C64 ? (adjoint ? 11 : 13) : (adjoint ? 11 : 12);
+    int tlog = 8, gen = 8, nt = 0;
+    bool chosen = false;  // the shape was measured for this size (table or circuit)
+    TuneChoice t;
+    if (tune_lookup(tune_key(n, dtype, adjoint ? "adjoint" : "forward"), &t) && t.set()) {
+        k = t.k;
+        tlog = t.tlog > 0 ? t.tlog : tlog;
+        gen = t.max_gen > 0 ? t.max_gen : gen;
+        nt = t.nt >= 0 ? t.nt : nt;
+        chosen = true;
     }
-    if (tlog == 0)
-        tlog = 8;
-    // small registers: fewer threads and smaller tiles keep more workgroups busy
-    if (n < k + 6) {
+    const int64_t k_set = tune_value(adjoint ? "adjoint_tile_qubits" : "tile_qubits", 0);
+    const int64_t t_set = tune_value("tile_threads_log2", 0);
+    if (k_set > 0 || t_set > 0) {
+        k = k_set > 0 ? (int)k_set : k;
+        tlog = t_set > 0 ? (int)t_set : 8;
+        chosen = false;
+    }
+    gen = (int)tune_value("max_gen_per_pass", gen);
+    nt = (int)tune_value("nontemporal", nt);
+    if (pin && pin->set()) {
+        k = pin->k;
+        tlog = pin->tlog > 0 ? pin->tlog : tlog;
+        gen = pin->max_gen > 0 ? pin->max_gen : gen;
+        nt = pin->nt >= 0 ? pin->nt : nt;
+        chosen = true;
+    }
+    // small states: fewer threads and smaller tiles keep more workgroups busy
+    if (!chosen && n < k + 6) {
         tlog = 6;
         k = adjoint ? 9 : (dtype == DMV_C64 ? 10 : 9);
     }
@@ -335,6 +350,8 @@ PlanParams tile_params(int n, int dtype, bool adjoint, StateLayout layout)
     pp.k = k;
     pp.tlog = tlog;
     pp.rlog = k - tlog;
+    pp.max_gen_per_pass = gen;
+    pp.nontemporal = nt != 0 && !adjoint ? 1 : 0;
     return pp;
 }
 
@@ -369,14 +386,14 @@ template <class T> static int upload(T *&dptr, size_t &cap, const std::vector<T>
 }
 
 int prepare_plan(DevicePlan &dp, uint64_t version, int n, int dtype, bool adjoint, const std::vector<HostOp> &ops,
-                 StateLayout layout)
+                 StateLayout layout, const TuneChoice *pin)
 {
     const int small_cap = (dtype == DMV_C64 ? kSmallMaxQubitsC64 : kSmallMaxQubitsC128) - (adjoint ? 1 : 0);
     const bool small = n <= small_cap;
-    PlanParams pp = small ? PlanParams{} : tile_params(n, dtype, adjoint, layout);
+    PlanParams pp = small ? PlanParams{} : tile_params(n, dtype, adjoint, layout, pin);
     const bool same_shape = dp.valid && dp.dtype == dtype && dp.small == small &&
                             (small || (dp.pp.k == pp.k && dp.pp.tlog == pp.tlog && dp.pp.max_gen_per_pass == pp.max_gen_per_pass &&
-                                       same_layout(dp.pp.layout, pp.layout)));
+                                       dp.pp.nontemporal == pp.nontemporal && same_layout(dp.pp.layout, pp.layout)));
     if (same_shape && dp.version == version && dp.epoch == tune_epoch())
         return DMV_OK;
     std::vector<HostOp> merged = merge_ops(ops);
@@ -551,7 +568,7 @@ static int forward_passes(DevicePlan &dp, dmv_state *s)
         prof_end(DMV_KC_SMALL, 2.0 * (double)s->bytes);
         return DMV_OK;
     }
-    const bool nt = tune_nontemporal();
+    const bool nt = dp.pp.nontemporal != 0;
     for (uint32_t p = 0; p < dp.plan.passes.size(); p++) {
         prof_begin(DMV_KC_TILE);
         DMV_TRY(launch_tile(s->dtype, dp.pp.rlog, dp.pp.tlog, nt, s->amp, s->n, s->layout, dp.d_passes, dp.d_phases, dp.d_ops, p));
@@ -635,11 +652,13 @@ int circuit_forward(dmv_circuit *c, dmv_state *s, int mode)
         return DMV_OK;
     }
     DMV_CHECK_ARG(mode == DMV_FORWARD_FUSED, "unknown forward mode %d", mode);
-    DMV_TRY(prepare_plan(c->fwd, c->version, c->n, s->dtype, false, forward_ops(c->gates), s->layout));
+    DMV_TRY(prepare_plan(c->fwd, c->version, c->n, s->dtype, false, forward_ops(c->gates), s->layout, &c->tune_fwd));
     if (tune_value("graph", 0) && !c->fwd.small)
         return forward_graph(c->fwd, s);
     return forward_passes(c->fwd, s);
 }
+
+int run_forward_plan(DevicePlan &dp, dmv_state *s) { return forward_passes(dp, s); }
 
 // -------------------------------------------------------- profiling access --
 int profile_enable(int on)

@@ -3,12 +3,26 @@
 
 #include <hip/hip_runtime.h>
 
+#include <string>
 #include <vector>
 
 #include "common.hpp"
 #include "plan.hpp"
 
 namespace dmv {
+
+// A launch-shape choice: the autotuner's result for one circuit, or an entry
+// of the process-wide table keyed by (GPU architecture, qubits, dtype, kind).
+// Fields <= 0 (nt < 0) are unset.
+struct TuneChoice {
+    int k = 0;        // tile qubits
+    int tlog = 0;     // log2 threads per workgroup
+    int max_gen = 0;  // general 2x2 ops per fused pass
+    int nt = -1;      // forward passes: nontemporal loads and stores (0/1)
+    int wgs = 0;      // batched kernel: workgroups per CU
+    double ms = 0.0;  // measured time of the choice
+    bool set() const { return k > 0; }
+};
 
 // A device buffer made of kSpreadPieces physical allocations mapped back to
 // back into one reserved address range (spread_alloc).
@@ -71,6 +85,7 @@ struct dmv_circuit {
     std::vector<dmv::Gate> gates;
     uint64_t version = 0;
     dmv::DevicePlan fwd, adj;
+    dmv::TuneChoice tune_fwd, tune_adj;  // set by the autotuner for this gate list
     double *d_partials = nullptr;   // adjoint partials of one pass: per (IP op, tile, wave)
     uint64_t cap_partials = 0;
     double *d_terms = nullptr;      // observable terms
@@ -98,12 +113,23 @@ uint64_t tune_epoch();
 // Layout a new n-qubit state gets (tunables page_fold_span_bits, page_fold).
 StateLayout default_layout(int n, int dtype);
 // Tile shape for a forward (adjoint = false) or reverse sweep on n qubits
-// over states of the given layout.
-PlanParams tile_params(int n, int dtype, bool adjoint, StateLayout layout);
+// over states of the given layout.  Each field comes from the first of: the
+// circuit's own choice `pin` (may be null), the tunables set by
+// dmv_tune_set or DMV_<NAME>, the tuning table, the defaults.
+PlanParams tile_params(int n, int dtype, bool adjoint, StateLayout layout, const TuneChoice *pin = nullptr);
 // Builds (if stale) and uploads the plan for `ops` into `dp`, for states of
 // the given layout.
 int prepare_plan(DevicePlan &dp, uint64_t version, int n, int dtype, bool adjoint,
-                 const std::vector<HostOp> &ops, StateLayout layout);
+                 const std::vector<HostOp> &ops, StateLayout layout, const TuneChoice *pin = nullptr);
+// Runs the passes of a prepared forward plan on s.
+int run_forward_plan(DevicePlan &dp, dmv_state *s);
+
+// The tuning table: "<arch>/n<qubits>/<c128|c64>/<forward|adjoint|batch>".
+// On first use it is filled from the JSON-lines file named by DMV_TUNE_CACHE
+// (later lines win); tune_store adds to the table and appends to that file.
+std::string tune_key(int n, int dtype, const char *kind);
+bool tune_lookup(const std::string &key, TuneChoice *out);
+void tune_store(const std::string &key, const TuneChoice &c);
 void free_plan(DevicePlan &dp);
 
 int state_create(int n, int dtype, dmv_state **out);
@@ -114,6 +140,11 @@ int state_set(dmv_state *s, const double *re, const double *im);
 int state_probs(dmv_state *s, double *host);
 
 int circuit_forward(dmv_circuit *c, dmv_state *s, int mode);
+// Adjoint gradient (dmv_circuit_adjoint) with the reverse-sweep plan `adj`
+// built under `pin`.
+int circuit_adjoint(dmv_circuit *c, dmv_state *psi, dmv_state *lam, const double *a, int n_pairs, const int *pi,
+                    const int *pj, const double *b, double *energy, double *grad, DevicePlan &adj,
+                    const TuneChoice *pin);
 
 // k samples of s into the device array d_out (uniforms d_u, or the seeded
 // stream when d_u is null), using only the state's workspace: no allocation.

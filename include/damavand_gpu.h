@@ -324,15 +324,46 @@ int dmv_copy_benchmark(uint64_t bytes, int repeats, double *ms_per_copy);
 int dmv_synchronize(void);
 
 /* ---- tuning ---- */
-/* Sets a tunable: "tile_qubits" (10..13), "tile_threads" (64 or 256),
- * "gate_pairs_per_thread" (1, 2, 4), "max_ops_per_pass", "nontemporal" (0/1),
- * "graph" (0/1: replay forward passes from a captured hipGraph). */
+/* Sets a tunable, e.g. "tile_qubits" / "adjoint_tile_qubits" (9..13),
+ * "tile_threads_log2" (6..9), "max_gen_per_pass", "nontemporal" (0/1),
+ * "gate_pairs_per_thread" (1, 2, 4), "batch_tile_qubits",
+ * "batch_threads_log2", "batch_wgs_per_cu", "graph" (0/1: replay forward
+ * passes from a captured hipGraph).  dmv_tune_get returns -1 when unset. */
 int dmv_tune_set(const char *name, int64_t value);
 int dmv_tune_get(const char *name, int64_t *value);
-/* Times the fused forward of `c` on `s` for each candidate tile shape and
- * keeps the fastest in the process-wide tuning table (and in the JSON file
- * named by DMV_TUNE_CACHE if set).  Writes the chosen tile qubits. */
+/* Autotuning of launch shapes.  The reference chooses grid and block sizes
+ * by an occupancy query before every launch (damavand-gpu/utils.cuh:62-133);
+ * here each shape is measured once, on the caller's workload, and reused:
+ *  - the choice is kept by the circuit (forward, adjoint) until
+ *    dmv_circuit_clear, and in a process-wide table keyed by
+ *    "<arch>/n<qubits>/<c128|c64>/<forward|adjoint|batch>";
+ *  - the table is read on first use from the JSON-lines file named by the
+ *    environment variable DMV_TUNE_CACHE (later lines win), and every new
+ *    choice is appended to that file, so later processes start from it;
+ *  - a plan takes each parameter from the first of: the circuit's own
+ *    choice, the tunables (dmv_tune_set / DMV_<NAME>), the table, the
+ *    defaults.
+ * The candidates run on scratch states the call allocates (the caller's
+ * states are left as they were) and no tunable is changed. */
+/* Times the fused forward of `c` for tile shapes x GEN budgets per pass
+ * (then streaming stores on the best); writes the chosen tile qubits. */
 int dmv_autotune(dmv_circuit *c, dmv_state *s, int *chosen_tile_qubits);
+/* The same for the reverse sweep of dmv_circuit_adjoint in `dtype`. */
+int dmv_autotune_adjoint(dmv_circuit *c, int dtype, int *chosen_tile_qubits);
+/* Times the batched kernel (tile shape x workgroups per CU) on this batch
+ * (arguments as dmv_batch_forward_adjoint) and keeps the fastest in the
+ * table under the qubits of its largest circuit. */
+int dmv_batch_autotune(int count, int dtype, const int *n, const int *gate_offset, const int *gate_kind,
+                       const int *gate_control, const int *gate_target, const double *gate_param,
+                       const int *coef_offset, const double *coef, const int *param_offset,
+                       int *chosen_tile_qubits);
+/* Empties the tuning table; the next lookup reads DMV_TUNE_CACHE again.
+ * Circuits keep their own choices. */
+int dmv_tune_forget(void);
+/* The tile shape the next forward (adjoint = 0) or reverse-sweep plan of `c`
+ * in `dtype` will use, after the precedence above.  No device needed. */
+int dmv_circuit_tile_shape(dmv_circuit *c, int dtype, int adjoint, int *tile_qubits, int *threads_log2,
+                           int *max_gen_per_pass, int *nontemporal);
 
 #ifdef __cplusplus
 }

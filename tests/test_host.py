@@ -267,3 +267,56 @@ def test_batch_rejects_bad_offsets_on_host():
     co[3] = -1
     assert _batch_call(b, coef_offset=co) == 1
     assert "coef_offset[3]" in N.last_error()
+
+
+_TUNE_PROBE = r"""
+import ctypes, json, sys
+sys.path.insert(0, sys.argv[1])
+from damavand_amd import _native as N
+from damavand_amd import workloads as wl
+from tests.libhelpers import NativeCircuit
+out = {}
+for n in (20, 21):
+    c = NativeCircuit(n, wl.layered_circuit(n, 1, 5))
+    for adj in (0, 1):
+        v = [ctypes.c_int(0) for _ in range(4)]
+        N.call("dmv_circuit_tile_shape", c.h, N.C128, adj, *[ctypes.byref(x) for x in v])
+        out[f"{n}/{adj}"] = [x.value for x in v]
+N.tune_set("tile_qubits", 10)
+v = [ctypes.c_int(0) for _ in range(4)]
+c = NativeCircuit(20, wl.layered_circuit(20, 1, 5))
+N.call("dmv_circuit_tile_shape", c.h, N.C128, 0, *[ctypes.byref(x) for x in v])
+out["20/0/tunable"] = [x.value for x in v]
+print(json.dumps(out))
+"""
+
+
+def test_tune_cache_is_read_back_with_precedence(tmp_path):
+    """The tuning table is filled from the DMV_TUNE_CACHE file on first use
+    (later lines win, unrelated lines are skipped), applies to circuits of
+    that size, and an explicit tunable takes precedence over it.  Runs in a
+    fresh process (the table is read once per process); no device needed."""
+    import json
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    cache = tmp_path / "tune.jsonl"
+    cache.write_text(
+        '{"key": "gfx950/n20/c128/forward", "tile_qubits": 12, "threads_log2": 9, "max_gen_per_pass": 6, '
+        '"nontemporal": 0, "wgs_per_cu": 0, "ms": 3.0}\n'
+        'not a cache line\n'
+        '{"key": "gfx950/n20/c128/forward", "tile_qubits": 11, "threads_log2": 7, "max_gen_per_pass": 12, '
+        '"nontemporal": 1, "wgs_per_cu": 0, "ms": 2.0}\n'
+        '{"key": "gfx950/n20/c128/adjoint", "tile_qubits": 10, "threads_log2": 8, "max_gen_per_pass": 16, '
+        '"nontemporal": 0, "wgs_per_cu": 0, "ms": 9.0}\n')
+    env = dict(os.environ, DMV_TUNE_CACHE=str(cache))
+    r = subprocess.run([sys.executable, "-c", _TUNE_PROBE, root], env=env, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = json.loads(r.stdout.strip().splitlines()[-1])
+    assert got["20/0"] == [11, 7, 12, 1]          # the later line for this key
+    assert got["20/1"] == [10, 8, 16, 0]          # reverse sweeps never stream
+    assert got["21/0"] == [12, 8, 8, 0]           # no entry: the defaults
+    assert got["21/1"] == [11, 8, 8, 0]
+    assert got["20/0/tunable"][0] == 10           # dmv_tune_set beats the table
