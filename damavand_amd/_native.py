@@ -72,6 +72,7 @@ _SIGNATURES = {
     "dmv_norm_sqr": (ctypes.c_int, [_vp, _c_dbl_p]),
     "dmv_expect_z": (ctypes.c_int, [_vp, _c_dbl_p]),
     "dmv_expect_zz": (ctypes.c_int, [_vp, _c_dbl_p, ctypes.c_int, _c_int_p, _c_int_p, _c_dbl_p, _c_dbl_p]),
+    "dmv_expect_diag": (ctypes.c_int, [_vp, ctypes.c_int, _c_int_p, _c_dbl_p, _c_dbl_p]),
     "dmv_inner_product": (ctypes.c_int, [_vp, _vp, _c_dbl_p]),
     "dmv_sample": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_uint64, _c_i64_p]),
     "dmv_sample_uniforms": (ctypes.c_int, [_vp, ctypes.c_int64, _c_dbl_p, _c_i64_p]),
@@ -89,6 +90,8 @@ _SIGNATURES = {
     "dmv_circuit_forward": (ctypes.c_int, [_vp, _vp, ctypes.c_int]),
     "dmv_circuit_adjoint": (ctypes.c_int, [_vp, _vp, _vp, _c_dbl_p, ctypes.c_int, _c_int_p, _c_int_p, _c_dbl_p,
                                            _c_dbl_p, _c_dbl_p]),
+    "dmv_circuit_adjoint_diag": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _c_int_p, _c_dbl_p, _c_dbl_p,
+                                                _c_dbl_p]),
     "dmv_circuit_plan_stats": (ctypes.c_int, [_vp, ctypes.c_int, _c_int_p, _c_int_p, _c_int_p, _c_dbl_p]),
     "dmv_circuit_plan_ops": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _c_int_p, _c_int_p, _c_u64_p, _c_dbl_p,
                                             _c_int_p, ctypes.c_int, _c_int_p]),

@@ -159,6 +159,14 @@ class StateVector:
                N.dptr(bb), ctypes.byref(out))
         return out.value
 
+    def expect_diag(self, wires: Sequence[int], table) -> float:
+        """Exact <H> for the diagonal H|x> = table[j(x)]|x>, j(x) the bits of x
+        at ``wires`` (wires[0] the least significant), up to 12 wires."""
+        w, t = _table_arrays(wires, table)
+        out = ctypes.c_double(0)
+        N.call("dmv_expect_diag", self._h, w.size, N.iptr(w), N.dptr(t), ctypes.byref(out))
+        return out.value
+
     def inner(self, other: "StateVector") -> complex:
         out = np.zeros(2)
         N.call("dmv_inner_product", self._h, other._h, N.dptr(out))
@@ -174,6 +182,14 @@ class StateVector:
         out = np.empty(u.size, dtype=np.int64)
         N.call("dmv_sample_uniforms", self._h, u.size, N.dptr(u), N.i64ptr(out))
         return out
+
+
+def _table_arrays(wires, table):
+    w = np.ascontiguousarray(wires, dtype=np.int32).reshape(-1)
+    t = np.ascontiguousarray(table, dtype=np.float64).reshape(-1)
+    if t.size != 1 << w.size:
+        raise ValueError(f"a table over {w.size} wires has {1 << w.size} entries, got {t.size}")
+    return w, t
 
 
 def _pairs_arrays(pairs, b):
@@ -390,6 +406,19 @@ class Circuit:
         e = ctypes.c_double(0)
         N.call("dmv_circuit_adjoint", self._c.h, psi.handle, lam.handle, N.dptr(a_arr), len(pi), N.iptr(pi),
                N.iptr(pj), N.dptr(bb), ctypes.byref(e), N.dptr(g))
+        return e.value, g[: self.num_parameters].copy()
+
+    def gradient_diag(self, wires: Sequence[int], table):
+        """Adjoint-method gradient of E(theta) = <0|U^dag H U|0> for the
+        diagonal H of StateVector.expect_diag.  Returns (E, dE/dtheta)."""
+        if self._scratch is None:
+            self._scratch = (StateVector(self.num_qubits, self.dtype), StateVector(self.num_qubits, self.dtype))
+        psi, lam = self._scratch
+        w, t = _table_arrays(wires, table)
+        g = np.empty(max(1, self.num_parameters))
+        e = ctypes.c_double(0)
+        N.call("dmv_circuit_adjoint_diag", self._c.h, psi.handle, lam.handle, w.size, N.iptr(w), N.dptr(t),
+               ctypes.byref(e), N.dptr(g))
         return e.value, g[: self.num_parameters].copy()
 
     # ------------------------------------------------------------ tuning --

@@ -90,9 +90,36 @@ static int upload_terms(double **buf, uint64_t *cap, int n, const double *a, int
     return DMV_OK;
 }
 
-int dmv::circuit_adjoint(dmv_circuit *c, dmv_state *psi, dmv_state *lam, const double *a, int n_pairs, const int *pi,
-                         const int *pj, const double *b, double *energy, double *grad, DevicePlan &dp,
-                         const TuneChoice *pin)
+// Uploads the table of a diagonal observable (2^n_wires doubles) and checks
+// its wires: distinct, in [0, n).
+static int upload_table(double **buf, uint64_t *cap, int n, int n_wires, const int *wires, const double *table,
+                        TableWires *W)
+{
+    DMV_CHECK_ARG(n_wires >= 0 && n_wires <= kMaxTableWires, "a table observable has 0 to %d wires", kMaxTableWires);
+    DMV_CHECK_ARG(n_wires == 0 || wires, "null wire array");
+    DMV_CHECK_ARG(table, "null table");
+    W->n = (uint32_t)n_wires;
+    uint64_t seen = 0;
+    for (int j = 0; j < n_wires; j++) {
+        DMV_CHECK_ARG(wires[j] >= 0 && wires[j] < n, "table observable wire %d out of range", wires[j]);
+        DMV_CHECK_ARG(!((seen >> wires[j]) & 1), "table observable wire %d repeated", wires[j]);
+        seen |= uint64_t(1) << wires[j];
+        W->q[j] = (uint8_t)wires[j];
+    }
+    const uint64_t need = uint64_t(1) << n_wires;
+    for (uint64_t i = 0; i < need; i++)
+        DMV_CHECK_ARG(std::isfinite(table[i]), "non-finite table entry %llu", (unsigned long long)i);
+    if (need > *cap) {
+        release(*buf);
+        *buf = nullptr;
+        DMV_TRY(alloc((void **)buf, need * sizeof(double)));
+        *cap = need;
+    }
+    return copy_h2d(*buf, table, need * sizeof(double));
+}
+
+int dmv::circuit_adjoint(dmv_circuit *c, dmv_state *psi, dmv_state *lam, const Observable &obs, double *energy,
+                         double *grad, DevicePlan &dp, const TuneChoice *pin)
 {
     CHECK_CIRCUIT(c);
     CHECK_STATE(psi);
@@ -107,12 +134,23 @@ int dmv::circuit_adjoint(dmv_circuit *c, dmv_state *psi, dmv_state *lam, const d
     DMV_TRY(state_reset(psi, 0, false));
     DMV_TRY(circuit_forward(c, psi, DMV_FORWARD_FUSED));
     // lambda = H psi, E = <psi|H|psi>
-    const double *da, *db;
-    const int *dpi, *dpj;
-    DMV_TRY(upload_terms(&c->d_terms, &c->cap_terms, c->n, a, n_pairs, pi, pj, b, &da, &db, &dpi, &dpj));
-    prof_begin(DMV_KC_REDUCE);
-    DMV_TRY(launch_apply_obs(psi->dtype, psi->amp, lam->amp, c->n, psi->layout, da, n_pairs, dpi, dpj, db, psi->ws, psi->scalars));
-    prof_end(DMV_KC_REDUCE, 3.0 * (double)psi->bytes);
+    if (obs.table) {
+        TableWires W;
+        DMV_TRY(upload_table(&c->d_terms, &c->cap_terms, c->n, obs.n_wires, obs.wires, obs.table, &W));
+        prof_begin(DMV_KC_REDUCE);
+        DMV_TRY(launch_obs_table(psi->dtype, psi->amp, lam->amp, c->n, psi->layout, c->d_terms, W, psi->ws,
+                                 psi->scalars));
+        prof_end(DMV_KC_REDUCE, 3.0 * (double)psi->bytes);
+    } else {
+        const double *da, *db;
+        const int *dpi, *dpj;
+        DMV_TRY(upload_terms(&c->d_terms, &c->cap_terms, c->n, obs.a, obs.n_pairs, obs.pi, obs.pj, obs.b, &da, &db,
+                             &dpi, &dpj));
+        prof_begin(DMV_KC_REDUCE);
+        DMV_TRY(launch_apply_obs(psi->dtype, psi->amp, lam->amp, c->n, psi->layout, da, obs.n_pairs, dpi, dpj, db,
+                                 psi->ws, psi->scalars));
+        prof_end(DMV_KC_REDUCE, 3.0 * (double)psi->bytes);
+    }
     // reverse sweep
     DMV_TRY(prepare_plan(dp, c->version, c->n, psi->dtype, true, adjoint_ops(c->gates), psi->layout, pin));
     if (dp.plan.n_slots != np) {
@@ -434,6 +472,19 @@ int dmv_expect_zz(dmv_state *s, const double *a, int n_pairs, const int *pi, con
     return copy_d2h(out, s->scalars, sizeof(double));
 }
 
+int dmv_expect_diag(dmv_state *s, int n_wires, const int *wires, const double *table, double *out)
+{
+    CHECK_STATE(s);
+    DMV_CHECK_ARG(out != nullptr, "null output");
+    DMV_TRY(ensure_device());
+    TableWires W;
+    DMV_TRY(upload_table(&s->terms, &s->cap_terms, s->n, n_wires, wires, table, &W));
+    prof_begin(DMV_KC_REDUCE);
+    DMV_TRY(launch_obs_table(s->dtype, s->amp, nullptr, s->n, s->layout, s->terms, W, s->ws, s->scalars));
+    prof_end(DMV_KC_REDUCE, (double)s->bytes);
+    return copy_d2h(out, s->scalars, sizeof(double));
+}
+
 int dmv_inner_product(dmv_state *a, dmv_state *b, double *out_re_im)
 {
     CHECK_STATE(a);
@@ -636,7 +687,24 @@ int dmv_circuit_forward(dmv_circuit *c, dmv_state *s, int mode)
 int dmv_circuit_adjoint(dmv_circuit *c, dmv_state *psi, dmv_state *lam, const double *a, int n_pairs, const int *pi,
                         const int *pj, const double *b, double *energy, double *grad)
 {
-    return circuit_adjoint(c, psi, lam, a, n_pairs, pi, pj, b, energy, grad, c->adj, &c->tune_adj);
+    Observable obs;
+    obs.a = a;
+    obs.n_pairs = n_pairs;
+    obs.pi = pi;
+    obs.pj = pj;
+    obs.b = b;
+    return circuit_adjoint(c, psi, lam, obs, energy, grad, c->adj, &c->tune_adj);
+}
+
+int dmv_circuit_adjoint_diag(dmv_circuit *c, dmv_state *psi, dmv_state *lam, int n_wires, const int *wires,
+                             const double *table, double *energy, double *grad)
+{
+    DMV_CHECK_ARG(table != nullptr, "null table");
+    Observable obs;
+    obs.n_wires = n_wires;
+    obs.wires = wires;
+    obs.table = table;
+    return circuit_adjoint(c, psi, lam, obs, energy, grad, c->adj, &c->tune_adj);
 }
 
 int dmv_circuit_plan_stats(dmv_circuit *c, int dtype, int *passes, int *phases, int *ops, double *hbm_bytes)

@@ -432,6 +432,70 @@ int launch_apply_obs(int dtype, const void *psi, void *lam, int n, StateLayout L
     return run_obs<true>(dtype, psi, lam, n, L, a, n_pairs, pi, pj, b, partials, out);
 }
 
+// Diagonal observable given by a table over up to kMaxTableWires qubits:
+// h(x) = table[sum_j bit_{wires[j]}(x) 2^j].  <H> partials per block, and
+// with WRITE also lambda = h * psi.
+template <class V, bool WRITE>
+__global__ void __launch_bounds__(RT) k_obs_table(const V *__restrict__ s, V *__restrict__ lam, uint32_t B,
+                                                  const double *__restrict__ table, TableWires W, double *partials,
+                                                  StateLayout L)
+{
+    __shared__ double red[RT / 64];
+    __shared__ double st[1 << kMaxTableWires];
+    for (uint32_t i = threadIdx.x; i < (1u << W.n); i += RT)
+        st[i] = table[i];
+    __syncthreads();
+    const uint64_t base = (uint64_t)blockIdx.x << B;
+    const uint64_t e0 = elem(base, L);
+    const uint32_t span = 1u << B;
+    double acc = 0.0;
+    for (uint32_t i = threadIdx.x; i < span; i += RT) {
+        const uint64_t x = base + i;
+        uint32_t idx = 0;
+        for (uint32_t j = 0; j < W.n; j++)
+            idx |= (uint32_t)((x >> W.q[j]) & 1u) << j;
+        const double h = st[idx];
+        const V v = s[e0 ^ i];
+        acc += h * norm2(v);
+        if constexpr (WRITE) {
+            V w;
+            w.x = v.x * h;
+            w.y = v.y * h;
+            lam[e0 ^ i] = w;
+        }
+    }
+    acc = block_sum<RT>(acc, red);
+    if (threadIdx.x == 0)
+        partials[blockIdx.x] = acc;
+}
+
+int launch_obs_table(int dtype, const void *psi, void *lam, int n, StateLayout L, const double *table,
+                     const TableWires &W, double *partials, double *out)
+{
+    DMV_CHECK_ARG(W.n <= (uint32_t)kMaxTableWires, "at most %d wires in a table observable", kMaxTableWires);
+    for (uint32_t j = 0; j < W.n; j++)
+        DMV_CHECK_ARG(W.q[j] < (uint32_t)n, "table observable wire out of range");
+    const int B = n < kBlockBits ? n : kBlockBits;
+    const uint64_t nblk = uint64_t(1) << (n - B);
+    if (dtype == DMV_C64) {
+        if (lam)
+            hipLaunchKernelGGL((k_obs_table<c64, true>), dim3((uint32_t)nblk), dim3(RT), 0, stream(), (const c64 *)psi,
+                               (c64 *)lam, (uint32_t)B, table, W, partials, L);
+        else
+            hipLaunchKernelGGL((k_obs_table<c64, false>), dim3((uint32_t)nblk), dim3(RT), 0, stream(),
+                               (const c64 *)psi, (c64 *)nullptr, (uint32_t)B, table, W, partials, L);
+    } else {
+        if (lam)
+            hipLaunchKernelGGL((k_obs_table<c128, true>), dim3((uint32_t)nblk), dim3(RT), 0, stream(),
+                               (const c128 *)psi, (c128 *)lam, (uint32_t)B, table, W, partials, L);
+        else
+            hipLaunchKernelGGL((k_obs_table<c128, false>), dim3((uint32_t)nblk), dim3(RT), 0, stream(),
+                               (const c128 *)psi, (c128 *)nullptr, (uint32_t)B, table, W, partials, L);
+    }
+    DMV_TRY(check_launch("table observable partials"));
+    return launch_sum(partials, nblk, out);
+}
+
 template <class V>
 __global__ void __launch_bounds__(RT) k_inner(const V *__restrict__ a, const V *__restrict__ b, uint32_t B,
                                               double *partials, StateLayout L)

@@ -78,6 +78,15 @@ int launch_expect_zz(int dtype, const void *state, int n, StateLayout L, const d
 int launch_apply_obs(int dtype, const void *psi, void *lam, int n, StateLayout L, const double *a, int n_pairs,
                      const int *pi, const int *pj, const double *b, double *partials, double *out);
 int launch_inner(int dtype, const void *a, const void *b, int n, StateLayout L, double *partials, double *out2);
+// Diagonal observable over up to kMaxTableWires qubits, h(x) = table[bits of x
+// at q[0..n)]: <H> into *out, and lambda = h * psi when lam is not null.
+constexpr int kMaxTableWires = 12;
+struct TableWires {
+    uint32_t n;
+    uint8_t q[kMaxTableWires];
+};
+int launch_obs_table(int dtype, const void *psi, void *lam, int n, StateLayout L, const double *table,
+                     const TableWires &W, double *partials, double *out);
 // Row sums: out[slot_map ? slot_map[b] : b] = sum_t partials[b * n_tiles + t]
 // for b < n_rows.
 int launch_slot_sums(const double *partials, int n_rows, uint64_t n_tiles, const int *slot_map, double *out);

@@ -140,11 +140,22 @@ int state_set(dmv_state *s, const double *re, const double *im);
 int state_probs(dmv_state *s, double *host);
 
 int circuit_forward(dmv_circuit *c, dmv_state *s, int mode);
+// The observable of an adjoint gradient: the Z/ZZ form
+// H = sum_q a_q Z_q + sum_t b_t Z_pi[t] Z_pj[t] when table is null, else the
+// diagonal H|x> = table[sum_j bit_{wires[j]}(x) 2^j] |x>.
+struct Observable {
+    const double *a = nullptr;
+    int n_pairs = 0;
+    const int *pi = nullptr, *pj = nullptr;
+    const double *b = nullptr;
+    int n_wires = 0;
+    const int *wires = nullptr;
+    const double *table = nullptr;
+};
 // Adjoint gradient (dmv_circuit_adjoint) with the reverse-sweep plan `adj`
 // built under `pin`.
-int circuit_adjoint(dmv_circuit *c, dmv_state *psi, dmv_state *lam, const double *a, int n_pairs, const int *pi,
-                    const int *pj, const double *b, double *energy, double *grad, DevicePlan &adj,
-                    const TuneChoice *pin);
+int circuit_adjoint(dmv_circuit *c, dmv_state *psi, dmv_state *lam, const Observable &obs, double *energy,
+                    double *grad, DevicePlan &adj, const TuneChoice *pin);
 
 // k samples of s into the device array d_out (uniforms d_u, or the seeded
 // stream when d_u is null), using only the state's workspace: no allocation.

@@ -309,6 +309,8 @@ int autotune_adjoint(dmv_circuit *c, int dtype, int *chosen)
     DMV_TRY(state_create(c->n, dtype, &psi.s));
     DMV_TRY(state_create(c->n, dtype, &lam.s));
     const std::vector<double> a(c->n, 1.0);  // H = sum_q Z_q: the sweep's cost does not depend on H
+    Observable obs;
+    obs.a = a.data();
     TuneChoice best;
     float best_ms = 1e30f;
     for (auto [k, tlog] : shapes(dtype, true)) {
@@ -323,8 +325,7 @@ int autotune_adjoint(dmv_circuit *c, int dtype, int *chosen)
             float ms = 0.f;
             DMV_TRY(time_runs(T,
                               [&] {
-                                  return circuit_adjoint(c, psi.s, lam.s, a.data(), 0, nullptr, nullptr, nullptr,
-                                                         nullptr, nullptr, P.dp, &cand);
+                                  return circuit_adjoint(c, psi.s, lam.s, obs, nullptr, nullptr, P.dp, &cand);
                               },
                               &ms));
             if (ms < best_ms) {

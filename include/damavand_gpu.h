@@ -204,6 +204,13 @@ int dmv_expect_z(dmv_state *s, double *out);
 /* Exact <H> for H = sum_q a[q] Z_q + sum_t b[t] Z_{pi[t]} Z_{pj[t]}; a has n entries (may be null). */
 int dmv_expect_zz(dmv_state *s, const double *a, int n_pairs, const int *pi, const int *pj,
                   const double *b, double *out);
+/* Exact <H> for a diagonal H over up to 12 qubits: H|x> = table[j(x)] |x>
+ * with j(x) = sum_k bit_{wires[k]}(x) 2^k; table has 2^n_wires entries and
+ * the wires are distinct.  Covers any product of Z on those qubits (table
+ * entries +-1), sums of them, and projectors (marginal probabilities).
+ * No reference counterpart (extract_expectation_values, circuit.rs:494-513,
+ * reads single qubits from samples). */
+int dmv_expect_diag(dmv_state *s, int n_wires, const int *wires, const double *table, double *out);
 /* <a|b> (complex, into out[2]). */
 int dmv_inner_product(dmv_state *a, dmv_state *b, double *out_re_im);
 /* k samples with the device splitmix64 stream of `seed`: the uniform of
@@ -259,6 +266,12 @@ int dmv_circuit_forward(dmv_circuit *c, dmv_state *s, int mode);
 int dmv_circuit_adjoint(dmv_circuit *c, dmv_state *psi, dmv_state *lambda, const double *a,
                         int n_pairs, const int *pi, const int *pj, const double *b,
                         double *energy, double *grad);
+
+/* dmv_circuit_adjoint with the diagonal observable of dmv_expect_diag.  With
+ * table = the upstream gradient of a vector of marginal probabilities, grad
+ * is that vector's vector-Jacobian product (one reverse sweep). */
+int dmv_circuit_adjoint_diag(dmv_circuit *c, dmv_state *psi, dmv_state *lambda, int n_wires, const int *wires,
+                             const double *table, double *energy, double *grad);
 
 /* Fused-plan statistics for a circuit on a state size/dtype: number of HBM
  * passes, total phases and ops, and algorithmic HBM bytes of one forward. */

@@ -59,16 +59,15 @@ class NativeCircuit:
         return p.value, ph.value, o.value, b.value
 
 
-def _hip_runtime():
-    """The HIP runtime instance the library itself loaded (found in this
-    process's mappings), so that raw copies see the library's allocations."""
-    N.lib()
-    path = next(line.split()[-1] for line in open("/proc/self/maps")
-                if "libamdhip64.so" in line and "/torch/" not in line)
-    hip = ctypes.CDLL(path)
-    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
-    hip.hipMemcpy.restype = ctypes.c_int
-    return hip
+def hip_memcpy():
+    """hipMemcpy of the HIP runtime the library itself is bound to (looked up
+    through the library's own handle, which searches its dependencies: the
+    system runtime, or torch's copy when torch was loaded first), so that
+    raw copies see the library's allocations."""
+    fn = N.lib().hipMemcpy
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    fn.restype = ctypes.c_int
+    return fn
 
 
 def amplitudes_at(state_handle, n, indices, amp_bytes=16):
@@ -83,12 +82,12 @@ def amplitudes_at(state_handle, n, indices, amp_bytes=16):
     idx = np.asarray(indices, dtype=np.int64)
     assert idx.min() >= 0 and idx.max() < (1 << n)
     el = element_map(idx, sh.value, bb.value, fo.value)
-    hip = _hip_runtime()
+    memcpy = hip_memcpy()
     N.synchronize()
     dt = np.complex128 if amp_bytes == 16 else np.complex64
     out = np.empty(idx.size, dtype=dt)
     one = np.empty(1, dtype=dt)
     for j, e in enumerate(el):
-        assert hip.hipMemcpy(one.ctypes.data, ptr.value + int(e) * amp_bytes, amp_bytes, 2) == 0
+        assert memcpy(one.ctypes.data, ptr.value + int(e) * amp_bytes, amp_bytes, 2) == 0
         out[j] = one[0]
     return out.astype(np.complex128)

@@ -17,6 +17,7 @@ import oracle
 from damavand_amd import _native as N
 from damavand_amd import workloads as wl
 from damavand_amd.circuit import Circuit, StateVector
+from tests.libhelpers import hip_memcpy
 from tests.stat_helpers import chi2_pvalue, marginal_pvalues
 
 pytestmark = pytest.mark.gpu
@@ -412,16 +413,12 @@ def test_hipgraph_replays(native):
 
 
 def _read_device(ptr, nbytes):
-    """Raw bytes of device memory, copied with the HIP runtime instance the
-    library itself loaded (found in this process's mappings)."""
-    N.lib()
-    path = next(line.split()[-1] for line in open("/proc/self/maps")
-                if "libamdhip64.so" in line and "/torch/" not in line)
-    hip = ctypes.CDLL(path)
-    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    """Raw bytes of device memory, copied with the HIP runtime the library
+    itself is bound to."""
+    memcpy = hip_memcpy()
     out = np.empty(nbytes, dtype=np.uint8)
     N.synchronize()
-    assert hip.hipMemcpy(out.ctypes.data, ptr, nbytes, 2) == 0  # hipMemcpyDeviceToHost
+    assert memcpy(out.ctypes.data, ptr, nbytes, 2) == 0  # hipMemcpyDeviceToHost
     return out
 
 
