@@ -1,8 +1,10 @@
 """ctypes binding of libdamavand-gpu.so (the C ABI of include/damavand_gpu.h).
 
-The library is loaded from this package directory.  There is no fallback:
-every call fails with an exception when the library is missing or when a
-dmv_* call reports an error.
+The library is loaded from this package directory, or from the path in
+DMV_LIBRARY (another build of the same library, e.g. the sanitizer build of
+``python -m damavand_amd.build --sanitize``).  There is no fallback: every
+call fails with an exception when the library is missing or when a dmv_*
+call reports an error.
 """
 from __future__ import annotations
 
@@ -12,7 +14,7 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libdamavand-gpu.so")
+LIB_PATH = os.environ.get("DMV_LIBRARY") or os.path.join(_HERE, "libdamavand-gpu.so")
 
 DMV_OK = 0
 C128, C64 = 0, 1

@@ -1,12 +1,17 @@
 """Builds libdamavand-gpu.so for gfx950 in-tree (damavand_amd/libdamavand-gpu.so).
 
-    python -m damavand_amd.build [--jobs N] [--verbose] [--incremental]
+    python -m damavand_amd.build [--jobs N] [--verbose] [--incremental] [--sanitize]
 
 Every source under damavand_amd/csrc is compiled with hipcc (objects under
 damavand_amd/_build/) and linked into one shared library exporting the C ABI
 declared in include/damavand_gpu.h.  No GPU is needed to build.  By default
 every unit is compiled; --incremental recompiles only units whose object is
 older than the unit or than any header under csrc/ and include/.
+
+--sanitize builds a second copy whose host code (planner, argument checks,
+runtime) is instrumented with AddressSanitizer and UndefinedBehaviorSanitizer
+(device code is not instrumented), into damavand_amd/_build_san/; load it with
+DMV_LIBRARY=<path> and clang's ASan runtime preloaded (tests/run_sanitized.py).
 """
 from __future__ import annotations
 
@@ -22,6 +27,11 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libdamavand-gpu.so")
+SAN_DIR = os.path.join(HERE, "_build_san")
+SAN_LIB = os.path.join(SAN_DIR, "libdamavand-gpu.so")
+# host-only instrumentation: each -fsanitize applies to the host pass alone
+SANITIZE = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+            "-Xarch_host", "-fno-omit-frame-pointer", "-Xarch_host", "-fno-sanitize-recover=undefined"]
 ARCH = os.environ.get("DMV_OFFLOAD_ARCH", "gfx950")
 
 COMMON = [
@@ -57,9 +67,9 @@ def sources():
     return out
 
 
-def _compile(src: str, verbose: bool) -> str:
-    obj = os.path.join(OBJ, os.path.basename(src) + ".o")
-    cmd = [hipcc()] + COMMON + ["-c", src, "-o", obj]
+def _compile(src: str, verbose: bool, sanitize: bool = False) -> str:
+    obj = os.path.join(SAN_DIR if sanitize else OBJ, os.path.basename(src) + ".o")
+    cmd = [hipcc()] + COMMON + (SANITIZE if sanitize else []) + ["-c", src, "-o", obj]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -97,25 +107,27 @@ def _up_to_date(src: str) -> bool:
     return all(t >= os.path.getmtime(d) for d in _deps(src) | {__file__})
 
 
-def build(jobs: int = 0, verbose: bool = False, incremental: bool = False) -> str:
-    os.makedirs(OBJ, exist_ok=True)
+def build(jobs: int = 0, verbose: bool = False, incremental: bool = False, sanitize: bool = False) -> str:
+    out_dir, lib = (SAN_DIR, SAN_LIB) if sanitize else (OBJ, LIB)
+    os.makedirs(out_dir, exist_ok=True)
     srcs = sources()
     jobs = jobs or min(8, os.cpu_count() or 1)
-    todo = [s for s in srcs if not (incremental and _up_to_date(s))]
+    todo = [s for s in srcs if not (incremental and not sanitize and _up_to_date(s))]
     # longest compiles first (the kernel instantiation units), so they overlap
     order = sorted(todo, key=lambda s: -os.path.getsize(s) if s.endswith(".hip") else 0)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        list(ex.map(lambda s: _compile(s, verbose), order))
-    objs = [os.path.join(OBJ, os.path.basename(s) + ".o") for s in srcs]
-    tmp = LIB + ".tmp"
-    cmd = [hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs
+        list(ex.map(lambda s: _compile(s, verbose, sanitize), order))
+    objs = [os.path.join(out_dir, os.path.basename(s) + ".o") for s in srcs]
+    tmp = lib + ".tmp"
+    link_san = ["-fsanitize=address,undefined", "-shared-libsan"] if sanitize else []
+    cmd = [hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}"] + link_san + ["-o", tmp] + objs
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"linking libdamavand-gpu.so failed:\n{r.stdout}\n{r.stderr}")
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib)
+    return lib
 
 
 def main() -> None:
@@ -123,8 +135,9 @@ def main() -> None:
     ap.add_argument("--jobs", type=int, default=0)
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--incremental", action="store_true")
+    ap.add_argument("--sanitize", action="store_true")
     a = ap.parse_args()
-    print(build(a.jobs, a.verbose, a.incremental))
+    print(build(a.jobs, a.verbose, a.incremental, a.sanitize))
 
 
 if __name__ == "__main__":
