@@ -52,21 +52,19 @@ inline int coalesce_bits(int dtype) { return dtype == DMV_C64 ? 4 : 3; }
 // ------------------------------------------------------ operation kinds --
 // A tile/gate operation.  GEN: 2x2 matrix on a register-resident target.
 // DIAG: diag(m00, m11) on any bit.  SWAP: X on a register-resident target.
-// IPX/IPY/IPZ: accumulate Im<lambda|P|psi> for P on the target (adjoint).
+// IP ops (adjoint): Im<lambda|A|psi> for A = ax X + ay Y + az Z on the
+// target, with (ax, ay, az) in m[0..2]; IPG on a register-resident target,
+// IPZ (ax = ay = 0) on any bit.
 enum OpKind : uint32_t {
     OP_GEN = 0,
     OP_DIAG = 1,
     OP_SWAP = 2,
-    OP_IPX = 4,
-    OP_IPY = 5,
+    OP_IPG = 4,
     OP_IPZ = 6,
 };
 
-inline DMV_HD bool op_needs_register(uint32_t kind)
-{
-    return kind == OP_GEN || kind == OP_SWAP || kind == OP_IPX || kind == OP_IPY;
-}
-inline DMV_HD bool op_is_ip(uint32_t kind) { return kind >= OP_IPX && kind <= OP_IPZ; }
+inline DMV_HD bool op_needs_register(uint32_t kind) { return kind == OP_GEN || kind == OP_SWAP || kind == OP_IPG; }
+inline DMV_HD bool op_is_ip(uint32_t kind) { return kind == OP_IPG || kind == OP_IPZ; }
 
 constexpr uint32_t kNoReg = 0xFFFFFFFFu;
 
@@ -85,7 +83,9 @@ constexpr uint32_t kNoReg = 0xFFFFFFFFu;
 //  * ctrl_thr: controls that are thread bits (mask over the thread id);
 //  * plain: a GEN op acting on every register (en = all) whose first pair
 //    members (registers with the partner mask's highest bit clear) all hold
-//    target bit 0, so the matrix applies without per-pair selects.
+//    target bit 0, so the matrix applies without per-pair selects; for an IP
+//    op: the previous op of the phase is an IP op of the same kind, target
+//    and masks, so its per-thread sums of X, Y and Z are reused.
 struct alignas(16) DevOp {
     uint32_t kind;
     uint32_t target;

@@ -129,14 +129,105 @@ static void dagger(const double m[8], double d[8])
     d[6] = m[6]; d[7] = -m[7];   // d11 = conj(m11)
 }
 
+static void matmul2(const double a[8], const double b[8], double c[8]);
+
+// Reverse sweep with every IP op moved to the start of its qubit's run of
+// uncontrolled one-qubit ops.  Within such a run the state at the IP's own
+// place is U psi, U the product of the run's ops before it (both psi and
+// lambda carry it), so Im<U lambda|P|U psi> = Im<lambda|U^dag P U|psi> with
+// U^dag P U = ax X + ay Y + az Z (Hermitian, traceless, real coefficients).
+// Ops on other qubits commute with all of this.  The run's ops then merge
+// into one 2x2 (merge_ops) and the run's IP ops share one pass over the
+// registers: an RY/RZ layer's reverse step is one GEN and one IP group
+// instead of two IP ops, a GEN and a diagonal.
+static std::vector<HostOp> hoist_ips(const std::vector<HostOp> &ops, int n)
+{
+    struct Run {
+        bool open = false;
+        double U[8];
+        int group = -1;
+    };
+    std::vector<Run> run(n);
+    std::vector<std::vector<HostOp>> groups;
+    std::vector<int> where;  // per output entry: group index, or -1 for an op
+    std::vector<HostOp> body;
+    auto open_run = [&](int q) {
+        Run &R = run[q];
+        if (R.open)
+            return;
+        R.open = true;
+        const double I[8] = {1, 0, 0, 0, 0, 0, 1, 0};
+        std::memcpy(R.U, I, sizeof(I));
+        R.group = (int)groups.size();
+        groups.emplace_back();
+        where.push_back(R.group);
+        body.emplace_back();
+    };
+    for (const HostOp &op : ops) {
+        const int q = op.target;
+        if (op_is_ip(op.kind)) {
+            open_run(q);
+            Run &R = run[q];
+            // A = U^dag P U with P = ax X + ay Y + az Z from op.m[0..2]
+            const double P[8] = {op.m[2], 0.0, op.m[0], -op.m[1], op.m[0], op.m[1], -op.m[2], 0.0};
+            double Ud[8], t[8], A[8];
+            Ud[0] = R.U[0]; Ud[1] = -R.U[1]; Ud[2] = R.U[4]; Ud[3] = -R.U[5];
+            Ud[4] = R.U[2]; Ud[5] = -R.U[3]; Ud[6] = R.U[6]; Ud[7] = -R.U[7];
+            matmul2(P, R.U, t);
+            matmul2(Ud, t, A);
+            HostOp ip = op;
+            std::memset(ip.m, 0, sizeof(ip.m));
+            ip.m[0] = A[4];  // A10 = ax + i ay
+            ip.m[1] = A[5];
+            ip.m[2] = A[0];  // A00 = az
+            groups[R.group].push_back(ip);
+            continue;
+        }
+        if (op.ctrl == 0) {
+            open_run(q);
+            Run &R = run[q];
+            double nu[8];
+            matmul2(op.m, R.U, nu);  // op acts after the run so far
+            std::memcpy(R.U, nu, sizeof(nu));
+        } else {
+            run[q].open = false;
+            for (int c = 0; c < n; c++)
+                if ((op.ctrl >> c) & 1)
+                    run[c].open = false;
+        }
+        where.push_back(-1);
+        body.push_back(op);
+    }
+    std::vector<HostOp> out;
+    out.reserve(ops.size());
+    for (size_t i = 0; i < where.size(); i++) {
+        if (where[i] < 0) {
+            out.push_back(body[i]);
+            continue;
+        }
+        std::vector<HostOp> &G = groups[where[i]];
+        // one kind per group, so that the kernel computes its sums once
+        bool general = false;
+        for (const HostOp &ip : G)
+            general = general || ip.m[0] != 0.0 || ip.m[1] != 0.0;
+        for (HostOp &ip : G) {
+            ip.kind = general ? OP_IPG : OP_IPZ;
+            out.push_back(ip);
+        }
+    }
+    return out;
+}
+
 std::vector<HostOp> adjoint_ops(const std::vector<Gate> &gates)
 {
     // parameter index of every gate
     std::vector<int> pidx(gates.size(), -1);
-    int np = 0;
-    for (size_t i = 0; i < gates.size(); i++)
+    int np = 0, n = 1;
+    for (size_t i = 0; i < gates.size(); i++) {
         if (gate_is_parametric(gates[i].kind))
             pidx[i] = np++;
+        n = std::max(n, std::max(gates[i].target, gates[i].control) + 1);
+    }
     std::vector<HostOp> out;
     out.reserve(2 * gates.size());
     for (size_t j = gates.size(); j-- > 0;) {
@@ -144,11 +235,13 @@ std::vector<HostOp> adjoint_ops(const std::vector<Gate> &gates)
         if (g.observable)
             continue;
         if (pidx[j] >= 0) {
+            // dE/dtheta = Im<lambda|P|psi> for e^{-i theta P / 2}, P on the target
             HostOp ip;
-            ip.kind = g.kind == DMV_GATE_RX ? OP_IPX : (g.kind == DMV_GATE_RY ? OP_IPY : OP_IPZ);
+            ip.kind = g.kind == DMV_GATE_RZ ? OP_IPZ : OP_IPG;
             ip.target = g.target;
             ip.ctrl = 0;
             std::memset(ip.m, 0, sizeof(ip.m));
+            ip.m[g.kind == DMV_GATE_RX ? 0 : (g.kind == DMV_GATE_RY ? 1 : 2)] = 1.0;
             ip.slot = pidx[j];
             out.push_back(ip);
         }
@@ -158,7 +251,7 @@ std::vector<HostOp> adjoint_ops(const std::vector<Gate> &gates)
         std::memcpy(op.m, d, sizeof(d));
         out.push_back(op);
     }
-    return out;
+    return tune_value("adjoint_hoist", 1) ? hoist_ips(out, n) : out;
 }
 
 // c = a * b for 2x2 complex matrices.
@@ -509,6 +602,14 @@ static int make_pass(uint64_t Q, const std::vector<HostOp> &taken, const PlanPar
                 op.plain = (op.en == all_regs && (op.fl & first_members(op.rmask, NR)) == 0) ? 1u : 0u;
             if (op_is_ip(h.kind) && h.slot + 1 > plan->n_slots)
                 plan->n_slots = h.slot + 1;
+            if (op_is_ip(h.kind) && plan->ops.size() > D.op_begin) {
+                // an IP op right after one of the same kind and masks reuses its sums
+                const DevOp &prev = plan->ops.back();
+                op.plain = prev.kind == op.kind && prev.target == op.target && prev.tbit == op.tbit &&
+                                   prev.rmask == op.rmask && prev.fl == op.fl
+                               ? 1u
+                               : 0u;
+            }
             plan->ops.push_back(op);
             plan->op_pass.push_back(pass_index);
         }
@@ -717,7 +818,15 @@ int validate_plan(const Plan &plan, const PlanParams &pp)
                     set_error("internal: op masks out of range");
                     return DMV_ERR_STATE;
                 }
-                if (op.plain && (op.kind != OP_GEN || op.en != regs || (op.fl & first_members(op.rmask, NR)) != 0)) {
+                if (op_is_ip(op.kind)) {
+                    const DevOp *prev = o > D.op_begin ? &plan.ops[o - 1] : nullptr;
+                    if (op.plain && !(prev && prev->kind == op.kind && prev->target == op.target &&
+                                      prev->tbit == op.tbit && prev->rmask == op.rmask && prev->fl == op.fl)) {
+                        set_error("internal: IP op reuses sums it has no predecessor for");
+                        return DMV_ERR_STATE;
+                    }
+                } else if (op.plain &&
+                           (op.kind != OP_GEN || op.en != regs || (op.fl & first_members(op.rmask, NR)) != 0)) {
                     set_error("internal: plain flag on an op that needs selects");
                     return DMV_ERR_STATE;
                 }

@@ -438,34 +438,39 @@ __device__ __forceinline__ void tile_body(typename Cplx<R>::T *__restrict__ stat
 }
 
 // --------------------------------------------------------------- adjoint --
-// Im<lambda|P|psi> over register pairs (p, p ^ M) for P = X or Y; for Y the
-// sign follows which member holds target bit 0.
+// Per-thread sums for an IP op on a register-bit target: over register pairs
+// (a, a ^ M) the parts of Im<lambda|X|psi>, Im<lambda|Y|psi> and
+// Im<lambda|Z|psi>.  With u0, u1 (v0, v1) the psi (lambda) amplitudes of
+// target bit 0 and 1: X gives Im(v0* u1) + Im(v1* u0), Y gives
+// -Re(v0* u1) + Re(v1* u0) and Z gives Im(v0* u0) - Im(v1* u1); the first
+// member a holds bit 1 where fl says so, which flips the signs of Y and Z.
 template <int M, class V, int NR>
-__device__ __forceinline__ double ip_pair_m(const V (&p)[NR], const V (&l)[NR], uint32_t kind, uint32_t fl)
+__device__ __forceinline__ void ip_pair_m(const V (&p)[NR], const V (&l)[NR], uint32_t fl, double &sx, double &sy,
+                                          double &sz)
 {
     constexpr int HB = highest_bit(M);
-    double acc = 0.0;
 #pragma unroll
     for (int a = 0; a < NR; a++) {
         if (a & HB)
             continue;
         const V pa = p[a], pb = p[a ^ M], la = l[a], lb = l[a ^ M];
-        if (kind == OP_IPX) {  // conj(la) pb + conj(lb) pa: symmetric in the pair
-            acc += (double)la.x * pb.y - (double)la.y * pb.x + (double)lb.x * pa.y - (double)lb.y * pa.x;
-        } else {  // (Y psi)_0 = -i psi_1, (Y psi)_1 = i psi_0
-            const double w = -((double)la.x * pb.x + (double)la.y * pb.y) + ((double)lb.x * pa.x + (double)lb.y * pa.y);
-            acc += ((fl >> a) & 1u) ? -w : w;
-        }
+        sx += (double)la.x * pb.y - (double)la.y * pb.x + (double)lb.x * pa.y - (double)lb.y * pa.x;
+        const double y = -((double)la.x * pb.x + (double)la.y * pb.y) + ((double)lb.x * pa.x + (double)lb.y * pa.y);
+        const double z = ((double)la.x * pa.y - (double)la.y * pa.x) - ((double)lb.x * pb.y - (double)lb.y * pb.x);
+        const bool f = (fl >> a) & 1u;
+        sy += f ? -y : y;
+        sz += f ? -z : z;
     }
-    return acc;
 }
 
+// This thread's sums for an IP op: (sx, sy, sz) for IPG; sz alone for IPZ,
+// whose target may be a thread bit.
 template <class V, int RLOG>
-__device__ __forceinline__ double ip_registers(const V (&p)[1 << RLOG], const V (&l)[1 << RLOG], const DevOp *op,
-                                               uint32_t tid)
+__device__ __forceinline__ void ip_sums(const V (&p)[1 << RLOG], const V (&l)[1 << RLOG], const DevOp *op,
+                                        uint32_t tid, double &sx, double &sy, double &sz)
 {
     constexpr int NR = 1 << RLOG;
-    double acc = 0.0;
+    sx = sy = sz = 0.0;
     if (op->kind == OP_IPZ) {
         const bool reg = op->tbit == kNoReg;
         const bool thread_one = !reg && ((tid >> op->tbit) & 1u);
@@ -474,21 +479,20 @@ __device__ __forceinline__ double ip_registers(const V (&p)[1 << RLOG], const V 
         for (int r = 0; r < NR; r++) {
             const bool one = reg ? ((ones >> r) & 1u) : thread_one;
             const double w = (double)l[r].x * p[r].y - (double)l[r].y * p[r].x;
-            acc += one ? -w : w;
+            sz += one ? -w : w;
         }
-        return acc;
+        return;
     }
     switch (op->rmask) {
 #define DMV_IP_CASE(M)                                                                             \
     case M:                                                                                        \
         if constexpr (M < NR)                                                                      \
-            acc = ip_pair_m<M>(p, l, op->kind, op->fl);                                            \
+            ip_pair_m<M>(p, l, op->fl, sx, sy, sz);                                                \
         break;
         DMV_MASK_CASES(DMV_IP_CASE)
 #undef DMV_IP_CASE
     default: break;
     }
-    return acc;
 }
 
 // One tile of a reverse-sweep pass over psi and lambda.  For every IP op the
@@ -513,6 +517,7 @@ __device__ __forceinline__ void tile_body_adjoint(typename Cplx<R>::T *__restric
     load_tile<false>(p, psi, gl, P->greg_ld, L);
     load_tile<false>(l, lam, gl, P->greg_ld, L);
     const DevLayout *cur = &P->lay_ld;
+    double sx = 0.0, sy = 0.0, sz = 0.0;  // sums of the current IP group
     for (uint32_t ph = P->phase_begin; ph < P->phase_end; ph++) {
         const DevPhase *D = phases + ph;
         if (D->transpose_in) {
@@ -523,13 +528,16 @@ __device__ __forceinline__ void tile_body_adjoint(typename Cplx<R>::T *__restric
         for (uint32_t o = D->op_begin; o < D->op_end; o++) {
             const DevOp *op = ops + o;
             if (op_is_ip(op->kind)) {
+                if (!op->plain)
+                    ip_sums<V, RLOG>(p, l, op, tid, sx, sy, sz);
+                const double v = op->m[0] * sx + op->m[1] * sy + op->m[2] * sz;
                 if constexpr (Sink::kPerWave) {
                     // one partial per wave: shuffles only, no workgroup barrier
-                    const double w = wave_sum(ip_registers<V, RLOG>(p, l, op, tid));
+                    const double w = wave_sum(v);
                     if ((tid & 63) == 0)
                         sink(op, w, tid >> 6);
                 } else {
-                    const double s = block_sum<NT>(ip_registers<V, RLOG>(p, l, op, tid), red);
+                    const double s = block_sum<NT>(v, red);
                     if (tid == 0)
                         sink(op, s, 0);
                 }
@@ -596,26 +604,21 @@ __device__ __forceinline__ void small_apply(V *s, uint32_t n, const DevOp *op)
     }
 }
 
-// This thread's part of Im<lambda|P|psi> for an IP op on a whole LDS state.
+// This thread's part of Im<lambda|A|psi>, A = ax X + ay Y + az Z on the
+// target, for an IP op on a whole LDS state.
 template <class V>
 __device__ __forceinline__ double small_ip(const V *sp, const V *sl, uint32_t n, const DevOp *op)
 {
     const uint32_t n_amp = 1u << n, t = op->target;
+    const double ax = op->m[0], ay = op->m[1], az = op->m[2];
     double acc = 0.0;
-    if (op->kind == OP_IPZ) {
-        for (uint32_t i = threadIdx.x; i < n_amp; i += blockDim.x) {
-            const double sg = (i >> t) & 1u ? -1.0 : 1.0;
-            acc += sg * ((double)sl[i].x * sp[i].y - (double)sl[i].y * sp[i].x);
-        }
-    } else {
-        for (uint32_t p = threadIdx.x; p < n_amp / 2; p += blockDim.x) {
-            const uint32_t i0 = (uint32_t)insert_zero_bit(p, t), i1 = i0 | (1u << t);
-            const V pa = sp[i0], pb = sp[i1], la = sl[i0], lb = sl[i1];
-            if (op->kind == OP_IPX)
-                acc += (double)la.x * pb.y - (double)la.y * pb.x + (double)lb.x * pa.y - (double)lb.y * pa.x;
-            else
-                acc += -((double)la.x * pb.x + (double)la.y * pb.y) + ((double)lb.x * pa.x + (double)lb.y * pa.y);
-        }
+    for (uint32_t q = threadIdx.x; q < n_amp / 2; q += blockDim.x) {
+        const uint32_t i0 = (uint32_t)insert_zero_bit(q, t), i1 = i0 | (1u << t);
+        const V u0 = sp[i0], u1 = sp[i1], v0 = sl[i0], v1 = sl[i1];
+        const double x = (double)v0.x * u1.y - (double)v0.y * u1.x + (double)v1.x * u0.y - (double)v1.y * u0.x;
+        const double y = -((double)v0.x * u1.x + (double)v0.y * u1.y) + ((double)v1.x * u0.x + (double)v1.y * u0.y);
+        const double z = ((double)v0.x * u0.y - (double)v0.y * u0.x) - ((double)v1.x * u1.y - (double)v1.y * u1.x);
+        acc += ax * x + ay * y + az * z;
     }
     return acc;
 }

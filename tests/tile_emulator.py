@@ -13,7 +13,7 @@ import numpy as np
 from damavand_amd import _native as N
 
 NO_REG = 0xFFFFFFFF
-OP_GEN, OP_DIAG, OP_SWAP, OP_IPX, OP_IPY, OP_IPZ = 0, 1, 2, 4, 5, 6
+OP_GEN, OP_DIAG, OP_SWAP, OP_IPG, OP_IPZ = 0, 1, 2, 4, 6
 
 
 def plan_json(handle, dtype=N.C128, adjoint=False, tile_qubits=0, tlog=0, max_gen=0):
@@ -120,31 +120,32 @@ class TileEmulator:
         m = op["m"]
         m00, m01, m10, m11 = complex(m[0], m[1]), complex(m[2], m[3]), complex(m[4], m[5]), complex(m[6], m[7])
         en, fl, rm = op["en"], op["fl"], op["rmask"]
-        if kind in (OP_IPX, OP_IPY, OP_IPZ):
+        if kind in (OP_IPG, OP_IPZ):
+            # Im<lambda|A|psi>, A = ax X + ay Y + az Z with (ax, ay, az) = m[0..2]
             p_, l_ = v, lam
-            acc = np.zeros(v.shape[0])
+            sx = np.zeros(v.shape[:2])
+            sy = np.zeros(v.shape[:2])
+            sz = np.zeros(v.shape[:2])
             if kind == OP_IPZ:
                 for r in range(self.NR):
                     if op["tbit"] == NO_REG:
                         sign = -1.0 if (rm >> r) & 1 else 1.0
                     else:
                         sign = np.where((self.tids >> op["tbit"]) & 1, -1.0, 1.0)
-                    w = (l_[:, :, r].real * p_[:, :, r].imag - l_[:, :, r].imag * p_[:, :, r].real)
-                    acc += (sign * w).sum(axis=1)
+                    sz += sign * (l_[:, :, r].real * p_[:, :, r].imag - l_[:, :, r].imag * p_[:, :, r].real)
             else:
                 hb = highest_bit(rm)
                 for a in range(self.NR):
                     if a & hb:
                         continue
                     b = a ^ rm
-                    pa, pb, la, lb = p_[:, :, a], p_[:, :, b], l_[:, :, a], l_[:, :, b]
-                    if kind == OP_IPX:
-                        w = la.real * pb.imag - la.imag * pb.real + lb.real * pa.imag - lb.imag * pa.real
-                    else:
-                        w = -(la.real * pb.real + la.imag * pb.imag) + (lb.real * pa.real + lb.imag * pa.imag)
-                        if (fl >> a) & 1:
-                            w = -w
-                    acc += w.sum(axis=1)
+                    # the member holding target bit 0 is a unless fl says otherwise
+                    i0, i1 = (b, a) if (fl >> a) & 1 else (a, b)
+                    u0, u1, v0, v1 = p_[:, :, i0], p_[:, :, i1], l_[:, :, i0], l_[:, :, i1]
+                    sx += (np.conj(v0) * u1 + np.conj(v1) * u0).imag
+                    sy += -(np.conj(v0) * u1).real + (np.conj(v1) * u0).real
+                    sz += (np.conj(v0) * u0).imag - (np.conj(v1) * u1).imag
+            acc = (m[0] * sx + m[1] * sy + m[2] * sz).sum(axis=1)
             sink(op["slot"], acc)
             return
         if kind == OP_DIAG:
@@ -200,7 +201,7 @@ class TileEmulator:
                     lv = self.transpose(lv, cur, ph["src_lswz"], ph["lay"])
             cur = ph["lay"]
             for op in ph["ops"]:
-                if op["kind"] in (OP_IPX, OP_IPY, OP_IPZ):
+                if op["kind"] in (OP_IPG, OP_IPZ):
                     self.apply(v, op, lam=lv, sink=sink)
                 else:
                     self.apply(v, op)
