@@ -804,6 +804,7 @@ int dmv_circuit_plan_json(dmv_circuit *c, int dtype, int adjoint, int tile_qubit
         pp.k = tile_qubits;
         pp.tlog = threads_log2 > 0 ? threads_log2 : (tile_qubits >= 10 ? 8 : 6);
         pp.rlog = pp.k - pp.tlog;
+        pp.half_lds = !adjoint && tune_value("half_lds", 1) != 0 && tile_half_exists(dtype, pp.rlog, pp.tlog) ? 1 : 0;
     }
     if (max_gen_per_pass > 0)
         pp.max_gen_per_pass = max_gen_per_pass;
@@ -835,7 +836,10 @@ int dmv_circuit_plan_json(dmv_circuit *c, int dtype, int adjoint, int tile_qubit
         json_layout(o, "lay_st", P.lay_st, pp);
         o += ",";
         json_u32s(o, "st_src_lswz", P.st_src_lswz, 1 << pp.rlog);
-        o += ",\"transpose_st\":" + std::to_string(P.transpose_st) + ",\"phases\":[";
+        o += ",";
+        json_u32s(o, "st_dst_lswz", P.st_dst_lswz, 1 << pp.rlog);
+        o += ",\"transpose_st\":" + std::to_string(P.transpose_st) + ",\"split\":" + std::to_string(P.split) +
+             ",\"lds_half\":" + std::to_string(P.lds_half) + ",\"phases\":[";
         for (uint32_t p = P.phase_begin; p < P.phase_end; p++) {
             const DevPhase &D = plan.phases[p];
             if (p != P.phase_begin)
@@ -844,6 +848,8 @@ int dmv_circuit_plan_json(dmv_circuit *c, int dtype, int adjoint, int tile_qubit
             json_layout(o, "lay", D.lay, pp);
             o += ",";
             json_u32s(o, "src_lswz", D.src_lswz, 1 << pp.rlog);
+            o += ",";
+            json_u32s(o, "dst_lswz", D.dst_lswz, 1 << pp.rlog);
             o += ",\"transpose_in\":" + std::to_string(D.transpose_in) + ",\"ops\":[";
             for (uint32_t q = D.op_begin; q < D.op_end; q++) {
                 const DevOp &op = plan.ops[q];

@@ -124,9 +124,14 @@ template <int G> inline DMV_HD uint32_t swz_index(uint32_t li)
     return li ^ (f & ((1u << G) - 1));
 }
 
+// LDS offsets of a transpose: register p is written at swz(thread part in the
+// source layout) ^ src_lswz[p] and register r read back at swz(thread part in
+// the destination layout) ^ dst_lswz[r].  In a pass with a split (DevPass),
+// both sides hold the offsets with the split bit removed.
 struct alignas(16) DevPhase {
     DevLayout lay;
     uint32_t src_lswz[kMaxRegs];  // LDS byte offsets of the incoming transpose's writes
+    uint32_t dst_lswz[kMaxRegs];  // ... and of its reads (lay.lswz unless split)
     uint32_t op_begin, op_end;
     uint32_t transpose_in;  // 1: go through LDS from the previous layout first
     uint32_t pad;
@@ -142,10 +147,27 @@ struct alignas(16) DevPass {
     uint32_t qpos[16];           // global qubit of local bit j, ascending
     DevLayout lay_ld, lay_st;
     uint32_t st_src_lswz[kMaxRegs];  // LDS byte offsets of the store transpose's writes
+    uint32_t st_dst_lswz[kMaxRegs];  // ... and of its reads
     uint32_t phase_begin, phase_end;
     uint32_t transpose_st;       // 1: go through LDS to lay_st before storing
     uint32_t k;
+    // Split transposes: one local bit w sits at the top thread bit of every
+    // layout of the pass, so the threads holding w = 0 and those holding
+    // w = 1 exchange amplitudes only among themselves.  Every transpose of
+    // the pass then runs as two rounds through half the tile's LDS, and its
+    // LDS byte offsets drop bit `split` (= w + log2 of the amplitude size,
+    // kept by the swizzle since w >= its group width).  0: no split.
+    uint32_t split;
+    uint32_t lds_half;  // 1: the pass fits kernels with half a tile of LDS
+    uint32_t pad[2];
 };
+
+// An LDS byte offset with bit b removed (the bits above it shift down).
+inline DMV_HD uint32_t drop_bit(uint32_t y, uint32_t b)
+{
+    const uint32_t lo = (1u << b) - 1;
+    return (y & lo) | ((y >> 1) & ~lo);
+}
 
 // ---------------------------------------------------------- bit helpers --
 inline DMV_HD uint64_t insert_zero_bit(uint64_t x, uint32_t pos)

@@ -12,21 +12,28 @@
 
 namespace dmv {
 
-int launch_tile(int dtype, int rlog, int tlog, bool nt, void *state, int n, StateLayout L, const DevPass *passes,
-                const DevPhase *phases, const DevOp *ops, uint32_t pass_idx)
+bool tile_half_exists(int dtype, int rlog, int tlog)
+{
+    return tlog == 8 && rlog == (dtype == DMV_C64 ? 5 : 4);
+}
+
+int launch_tile(int dtype, int rlog, int tlog, bool nt, bool half, void *state, int n, StateLayout L,
+                const DevPass *passes, const DevPhase *phases, const DevOp *ops, uint32_t pass_idx)
 {
     DMV_CHECK_ARG(n >= rlog + tlog && n - (rlog + tlog) <= 31, "tile shape does not fit %d qubits", n);
+    DMV_CHECK_ARG(!half || tile_half_exists(dtype, rlog, tlog), "no half-LDS tile kernel for rlog %d tlog %d", rlog,
+                  tlog);
     int rc = kNotHandled;
     if (dtype == DMV_C128) {
-        rc = launch_tile_c128_main(rlog, tlog, nt, state, n, L, passes, phases, ops, pass_idx);
-        if (rc == kNotHandled)
+        rc = launch_tile_c128_main(rlog, tlog, nt, half, state, n, L, passes, phases, ops, pass_idx);
+        if (rc == kNotHandled && !half)
             rc = launch_tile_c128_other(rlog, tlog, nt, state, n, L, passes, phases, ops, pass_idx);
     } else {
-        rc = launch_tile_c64(rlog, tlog, nt, state, n, L, passes, phases, ops, pass_idx);
+        rc = launch_tile_c64(rlog, tlog, nt, half, state, n, L, passes, phases, ops, pass_idx);
     }
     if (rc != kNotHandled)
         return rc;
-    set_error("no tile kernel for dtype %d rlog %d tlog %d", dtype, rlog, tlog);
+    set_error("no tile kernel for dtype %d rlog %d tlog %d%s", dtype, rlog, tlog, half ? " (half LDS)" : "");
     return DMV_ERR_ARG;
 }
 

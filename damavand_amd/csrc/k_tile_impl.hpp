@@ -6,25 +6,27 @@
 
 namespace dmv {
 
-template <class R, int RLOG, int TLOG, bool NT>
+// HALF: half a tile of LDS (split transposes), so that three workgroups of
+// the 64 KiB tiles share a CU instead of two.
+template <class R, int RLOG, int TLOG, bool NT, bool HALF>
 __global__ void __launch_bounds__(1 << TLOG)
 k_tile(typename Cplx<R>::T *__restrict__ state, const DevPass *__restrict__ passes,
        const DevPhase *__restrict__ phases, const DevOp *__restrict__ ops, uint32_t pass_idx, uint32_t xcd_log2,
        StateLayout L)
 {
     using V = typename Cplx<R>::T;
-    __shared__ V lds[1 << (RLOG + TLOG)];
+    __shared__ V lds[1 << (RLOG + TLOG - (HALF ? 1 : 0))];
     // Workgroups are dealt to the XCDs round-robin.  With xcd_log2 = x > 0
     // (grid a multiple of 2^x), XCD j takes the j-th contiguous 2^-x share of
     // the tiles, so the tiles one XCD runs at a time are neighbours in memory.
     const uint32_t b = blockIdx.x;
     const uint64_t tile = xcd_log2 ? (uint64_t)(b & ((1u << xcd_log2) - 1)) * (gridDim.x >> xcd_log2) + (b >> xcd_log2)
                                    : (uint64_t)b;
-    tile_body<R, RLOG, TLOG, NT>(state, passes + pass_idx, phases, ops, tile, lds, L);
+    tile_body<R, RLOG, TLOG, NT, HALF>(state, passes + pass_idx, phases, ops, tile, lds, L);
 }
 
 // ------------------------------------------------------------- launchers --
-template <class R, int RLOG, int TLOG>
+template <class R, int RLOG, int TLOG, bool HALF>
 static int run_tile(bool nt, void *state, int n, StateLayout L, const DevPass *passes, const DevPhase *phases, const DevOp *ops,
                     uint32_t pass_idx)
 {
@@ -33,24 +35,24 @@ static int run_tile(bool nt, void *state, int n, StateLayout L, const DevPass *p
     const int64_t xl = tune_value("tile_xcd_log2", 3);  // 8 XCDs per MI355X
     const uint32_t xcd_log2 = (xl > 0 && xl <= 4 && tiles >= (uint64_t(1) << xl)) ? (uint32_t)xl : 0u;
     if (nt)
-        hipLaunchKernelGGL((k_tile<R, RLOG, TLOG, true>), dim3((uint32_t)tiles), dim3(1 << TLOG), 0, stream(),
+        hipLaunchKernelGGL((k_tile<R, RLOG, TLOG, true, HALF>), dim3((uint32_t)tiles), dim3(1 << TLOG), 0, stream(),
                            (V *)state, passes, phases, ops, pass_idx, xcd_log2, L);
     else
-        hipLaunchKernelGGL((k_tile<R, RLOG, TLOG, false>), dim3((uint32_t)tiles), dim3(1 << TLOG), 0, stream(),
-                           (V *)state, passes, phases, ops, pass_idx, xcd_log2, L);
+        hipLaunchKernelGGL((k_tile<R, RLOG, TLOG, false, HALF>), dim3((uint32_t)tiles), dim3(1 << TLOG), 0,
+                           stream(), (V *)state, passes, phases, ops, pass_idx, xcd_log2, L);
     return check_launch("tile pass");
 }
 
 // Shape-specific launchers (one translation unit each); they return
-// kNotHandled when the shape is not theirs.
+// kNotHandled when the shape (or its half-LDS variant) is not theirs.
 constexpr int kNotHandled = -1;
-int launch_tile_c128_main(int rlog, int tlog, bool nt, void *state, int n, StateLayout L, const DevPass *passes,
-                          const DevPhase *phases, const DevOp *ops, uint32_t pass_idx);
+int launch_tile_c128_main(int rlog, int tlog, bool nt, bool half, void *state, int n, StateLayout L,
+                          const DevPass *passes, const DevPhase *phases, const DevOp *ops, uint32_t pass_idx);
 int launch_tile_c128_other(int rlog, int tlog, bool nt, void *state, int n, StateLayout L, const DevPass *passes,
                            const DevPhase *phases, const DevOp *ops, uint32_t pass_idx);
 int launch_tile_c128_wave(int rlog, int tlog, bool nt, void *state, int n, StateLayout L, const DevPass *passes,
                           const DevPhase *phases, const DevOp *ops, uint32_t pass_idx);
-int launch_tile_c64(int rlog, int tlog, bool nt, void *state, int n, StateLayout L, const DevPass *passes,
+int launch_tile_c64(int rlog, int tlog, bool nt, bool half, void *state, int n, StateLayout L, const DevPass *passes,
                     const DevPhase *phases, const DevOp *ops, uint32_t pass_idx);
 
 }  // namespace dmv

@@ -9,9 +9,12 @@
 namespace dmv {
 
 // One fused pass of a tile plan over a state of n qubits in layout L (every
-// state launcher below takes the state's layout L).
-int launch_tile(int dtype, int rlog, int tlog, bool nontemporal, void *state, int n, StateLayout L,
+// state launcher below takes the state's layout L).  half: the kernel with
+// half a tile of LDS, for passes whose transposes are all split.
+int launch_tile(int dtype, int rlog, int tlog, bool nontemporal, bool half, void *state, int n, StateLayout L,
                 const DevPass *passes, const DevPhase *phases, const DevOp *ops, uint32_t pass_idx);
+// Whether a half-LDS tile kernel exists for the shape.
+bool tile_half_exists(int dtype, int rlog, int tlog);
 // Reverse-sweep tile pass over psi and lambda; the IP ops of the pass write
 // one partial per (IP op, tile, wave) into
 // partials[(ip_pos * tiles + tile) * waves + wave].

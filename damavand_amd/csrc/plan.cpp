@@ -338,7 +338,8 @@ static uint64_t deposit(uint64_t x, const uint32_t *pos, int nbits)
 // must not be register bits); otherwise thread bits 0..swz_bits-1 get local
 // bits with distinct residues mod swz_bits, which keeps 128-bit LDS accesses
 // of consecutive lanes in distinct banks under the swizzle of the kernels.
-static DevLayout make_layout(uint32_t rs, const PlanParams &pp, bool coalesced)
+// top >= 0: local bit `top` (not in rs) takes the highest thread bit.
+static DevLayout make_layout(uint32_t rs, const PlanParams &pp, bool coalesced, int top)
 {
     DevLayout L;
     std::memset(&L, 0, sizeof(L));
@@ -351,7 +352,7 @@ static DevLayout make_layout(uint32_t rs, const PlanParams &pp, bool coalesced)
         L.lreg[r] = (uint32_t)deposit((uint64_t)r, regbits, nr);
     std::vector<int> tb;
     for (int b = 0; b < pp.k; b++)
-        if (!((rs >> b) & 1))
+        if (!((rs >> b) & 1) && b != top)
             tb.push_back(b);
     std::vector<int> order;
     if (coalesced) {
@@ -368,6 +369,8 @@ static DevLayout make_layout(uint32_t rs, const PlanParams &pp, bool coalesced)
     for (int b : tb)
         if (std::find(order.begin(), order.end(), b) == order.end())
             order.push_back(b);
+    if (top >= 0)
+        order.push_back(top);
     for (int j = 0; j < pp.tlog; j++)
         L.thr[j] = (uint32_t)order[j];
     const uint32_t elem = pp.swz_bits == 3 ? 16u : 8u;
@@ -406,13 +409,15 @@ static bool layouts_equal(const DevLayout &a, const DevLayout &b, const PlanPara
 }
 
 // Pads a register set to rlog bits, preferring the highest local bits that
-// are not kept in lanes for coalescing.
-static uint32_t pad_regs(uint32_t rs, const PlanParams &pp)
+// are not kept in lanes for coalescing, and never taking the bits of `avoid`.
+static uint32_t pad_regs(uint32_t rs, const PlanParams &pp, uint32_t avoid)
 {
     for (int b = pp.k - 1; b >= pp.low_bits && popcount64(rs) < pp.rlog; b--)
-        rs |= 1u << b;
+        if (!((avoid >> b) & 1))
+            rs |= 1u << b;
     for (int b = 0; b < pp.k && popcount64(rs) < pp.rlog; b++)
-        rs |= 1u << b;
+        if (!((avoid >> b) & 1))
+            rs |= 1u << b;
     return rs;
 }
 
@@ -491,8 +496,21 @@ static int make_pass(uint64_t Q, const std::vector<HostOp> &taken, const PlanPar
         phs.push_back(std::move(ph));
     }
     const uint32_t lowmask = (1u << pp.low_bits) - 1;
+    // Split transposes: the highest local bit above the lane-coalesced and
+    // swizzle-group bits that no op of the pass needs in a register stays a
+    // thread bit, at the top, in every layout of the pass (wave-uniform halves).
+    int top = -1;
+    if (pp.half_lds && pp.tlog >= 7) {
+        uint32_t targets = 0;
+        for (const Ph &ph : phs)
+            targets |= ph.rs;
+        for (int b = pp.k - 1; b >= std::max(pp.low_bits, pp.swz_bits) && top < 0; b--)
+            if (!((targets >> b) & 1))
+                top = b;
+    }
+    const uint32_t avoid = top >= 0 ? 1u << top : 0u;
     for (Ph &ph : phs)
-        ph.rs = pad_regs(ph.rs, pp);
+        ph.rs = pad_regs(ph.rs, pp, avoid);
 
     // Layouts: load/store layouts keep the low qubits in lanes.  By default
     // the first/last phase's layout is reused when it keeps the low qubits in
@@ -501,11 +519,14 @@ static int make_pass(uint64_t Q, const std::vector<HostOp> &taken, const PlanPar
     // ascending order and registers its highest, and a phase whose register
     // set differs pays an LDS transpose instead (measured slower overall on
     // the gapped state layout).
-    const uint32_t default_rs = pad_regs(0u, pp);
+    const uint32_t default_rs = pad_regs(0u, pp, avoid);
     const bool mem_order = tune_value("memory_layouts", 0) != 0;
     const bool first_coalesced = (phs.front().rs & lowmask) == 0;
     const bool last_coalesced = (phs.back().rs & lowmask) == 0;
-    P.lay_ld = make_layout(first_coalesced && !mem_order ? phs.front().rs : default_rs, pp, true);
+    P.lay_ld = make_layout(first_coalesced && !mem_order ? phs.front().rs : default_rs, pp, true, top);
+    const uint32_t elem_log2 = pp.swz_bits == 3 ? 4u : 3u;
+    P.split = top >= 0 ? (uint32_t)top + elem_log2 : 0u;
+    auto lds_off = [&](uint32_t y) { return P.split ? drop_bit(y, P.split) : y; };
     P.phase_begin = (uint32_t)plan->phases.size();
     const int pass_index = (int)plan->passes.size();
     const int NR = 1 << pp.rlog;
@@ -532,11 +553,13 @@ static int make_pass(uint64_t Q, const std::vector<HostOp> &taken, const PlanPar
         std::memset(&D, 0, sizeof(D));
         const bool coal = (p == 0 && first_coalesced) ||
                           (p + 1 == phs.size() && last_coalesced && !(p == 0 && !first_coalesced));
-        D.lay = make_layout(phs[p].rs, pp, coal && (phs[p].rs & lowmask) == 0);
+        D.lay = make_layout(phs[p].rs, pp, coal && (phs[p].rs & lowmask) == 0, top);
         D.transpose_in = layouts_equal(D.lay, prev, pp) ? 0u : 1u;
         if (D.transpose_in) {
-            for (int q = 0; q < NR; q++)
-                D.src_lswz[q] = prev.lswz[tab[q]];
+            for (int q = 0; q < NR; q++) {
+                D.src_lswz[q] = lds_off(prev.lswz[tab[q]]);
+                D.dst_lswz[q] = lds_off(D.lay.lswz[q]);
+            }
             reset_tab();
         }
         D.op_begin = (uint32_t)plan->ops.size();
@@ -619,15 +642,23 @@ static int make_pass(uint64_t Q, const std::vector<HostOp> &taken, const PlanPar
     }
     P.phase_end = (uint32_t)plan->phases.size();
     const DevLayout last = prev;
-    const bool store_direct = mem_order ? layouts_equal(last, make_layout(default_rs, pp, true), pp)
-                                        : last_coalesced && layouts_equal(last, make_layout(phs.back().rs, pp, true), pp);
-    P.lay_st = store_direct ? last : make_layout(default_rs, pp, true);
+    const bool store_direct = mem_order ? layouts_equal(last, make_layout(default_rs, pp, true, top), pp)
+                                        : last_coalesced &&
+                                              layouts_equal(last, make_layout(phs.back().rs, pp, true, top), pp);
+    P.lay_st = store_direct ? last : make_layout(default_rs, pp, true, top);
     P.transpose_st = layouts_equal(P.lay_st, last, pp) ? 0u : 1u;
     if (P.transpose_st) {
-        for (int q = 0; q < NR; q++)
-            P.st_src_lswz[q] = last.lswz[tab[q]];
+        for (int q = 0; q < NR; q++) {
+            P.st_src_lswz[q] = lds_off(last.lswz[tab[q]]);
+            P.st_dst_lswz[q] = lds_off(P.lay_st.lswz[q]);
+        }
         reset_tab();
     }
+    // a pass without transposes needs no LDS at all
+    bool any_transpose = P.transpose_st != 0;
+    for (uint32_t p = P.phase_begin; p < (uint32_t)plan->phases.size(); p++)
+        any_transpose = any_transpose || plan->phases[p].transpose_in;
+    P.lds_half = (P.split || !any_transpose) && pp.half_lds ? 1u : 0u;
     // global offsets of the load/store layouts (the store through the final map)
     for (int r = 0; r < NR; r++) {
         // element offsets: the kernels add them to the element of tile base | thread offset
@@ -744,16 +775,39 @@ static int check_layout(const DevLayout &L, const PlanParams &pp, const char *wh
     return DMV_OK;
 }
 
-// A transpose's write offsets must be a permutation of its source layout's.
-static int check_offsets(const uint32_t *src, const DevLayout &L, const PlanParams &pp, const char *what)
+// A transpose's write offsets must be a permutation of its source layout's
+// and its read offsets its destination layout's, with the split bit removed
+// in a split pass.  There, the split local bit must be the top thread bit of
+// both layouts (so each half of the threads exchanges amplitudes only within
+// itself) and the offsets must fit half a tile.
+static int check_offsets(const uint32_t *src, const DevLayout &A, const uint32_t *dst, const DevLayout &B,
+                         uint32_t split, const PlanParams &pp, const char *what)
 {
     const int NR = 1 << pp.rlog;
-    std::vector<uint32_t> a(src, src + NR), b(L.lswz, L.lswz + NR);
+    const uint32_t elem_log2 = pp.swz_bits == 3 ? 4u : 3u;
+    const uint32_t limit = (1u << (pp.k + (int)elem_log2)) >> (split ? 1 : 0);
+    std::vector<uint32_t> a(src, src + NR), b(NR), c(dst, dst + NR), d(NR);
+    for (int r = 0; r < NR; r++) {
+        b[r] = split ? drop_bit(A.lswz[r], split) : A.lswz[r];
+        d[r] = split ? drop_bit(B.lswz[r], split) : B.lswz[r];
+        if (d[r] != c[r] || a[r] >= limit) {
+            set_error("internal: %s read offsets inconsistent", what);
+            return DMV_ERR_STATE;
+        }
+    }
     std::sort(a.begin(), a.end());
     std::sort(b.begin(), b.end());
     if (a != b) {
         set_error("internal: %s offsets are not a permutation of the layout", what);
         return DMV_ERR_STATE;
+    }
+    if (split) {
+        const uint32_t w = split - elem_log2;
+        if (split < elem_log2 || (int)w < pp.swz_bits || (int)w >= pp.k || pp.tlog < 7 ||
+            A.thr[pp.tlog - 1] != w || B.thr[pp.tlog - 1] != w) {
+            set_error("internal: %s split bit is not the top thread bit of both layouts", what);
+            return DMV_ERR_STATE;
+        }
     }
     return DMV_OK;
 }
@@ -800,8 +854,8 @@ int validate_plan(const Plan &plan, const PlanParams &pp)
                 return DMV_ERR_STATE;
             }
             if (D.transpose_in)
-                DMV_TRY(check_offsets(D.src_lswz, p == P.phase_begin ? P.lay_ld : plan.phases[p - 1].lay, pp,
-                                      "transpose source"));
+                DMV_TRY(check_offsets(D.src_lswz, p == P.phase_begin ? P.lay_ld : plan.phases[p - 1].lay,
+                                      D.dst_lswz, D.lay, P.split, pp, "phase transpose"));
             for (uint32_t o = D.op_begin; o < D.op_end; o++) {
                 const DevOp &op = plan.ops[o];
                 const int NR = 1 << pp.rlog;
@@ -853,7 +907,19 @@ int validate_plan(const Plan &plan, const PlanParams &pp)
             }
         }
         if (P.transpose_st)
-            DMV_TRY(check_offsets(P.st_src_lswz, plan.phases[P.phase_end - 1].lay, pp, "store transpose source"));
+            DMV_TRY(check_offsets(P.st_src_lswz, plan.phases[P.phase_end - 1].lay, P.st_dst_lswz, P.lay_st, P.split,
+                                  pp, "store transpose"));
+        bool any_transpose = P.transpose_st != 0;
+        for (uint32_t p = P.phase_begin; p < P.phase_end; p++)
+            any_transpose = any_transpose || plan.phases[p].transpose_in;
+        if ((P.lds_half || P.split) && !pp.half_lds) {
+            set_error("internal: split transposes in a plan for full-LDS kernels");
+            return DMV_ERR_STATE;
+        }
+        if (P.lds_half && any_transpose && !P.split) {
+            set_error("internal: half-LDS pass with a full transpose");
+            return DMV_ERR_STATE;
+        }
     }
     return DMV_OK;
 }

@@ -55,6 +55,7 @@ struct PlanParams {
     int swz_bits;   // LDS swizzle group (3 for 16-byte, 4 for 8-byte amplitudes)
     int max_gen_per_pass;  // compute budget of a pass, in general 2x2 ops
     int nontemporal = 0;   // forward passes: streaming (nontemporal) loads and stores
+    int half_lds = 0;      // forward passes: split transposes where a pass allows (half-LDS kernels exist)
     StateLayout layout = kPlainLayout;  // element map of the states the plan runs on
 };
 

@@ -352,6 +352,8 @@ PlanParams tile_params(int n, int dtype, bool adjoint, StateLayout layout, const
     pp.rlog = k - tlog;
     pp.max_gen_per_pass = gen;
     pp.nontemporal = nt != 0 && !adjoint ? 1 : 0;
+    // forward passes split their transposes where the shape has half-LDS kernels
+    pp.half_lds = !adjoint && tune_value("half_lds", 1) != 0 && tile_half_exists(dtype, pp.rlog, pp.tlog) ? 1 : 0;
     return pp;
 }
 
@@ -393,7 +395,8 @@ int prepare_plan(DevicePlan &dp, uint64_t version, int n, int dtype, bool adjoin
     PlanParams pp = small ? PlanParams{} : tile_params(n, dtype, adjoint, layout, pin);
     const bool same_shape = dp.valid && dp.dtype == dtype && dp.small == small &&
                             (small || (dp.pp.k == pp.k && dp.pp.tlog == pp.tlog && dp.pp.max_gen_per_pass == pp.max_gen_per_pass &&
-                                       dp.pp.nontemporal == pp.nontemporal && same_layout(dp.pp.layout, pp.layout)));
+                                       dp.pp.nontemporal == pp.nontemporal && dp.pp.half_lds == pp.half_lds &&
+                                       same_layout(dp.pp.layout, pp.layout)));
     if (same_shape && dp.version == version && dp.epoch == tune_epoch())
         return DMV_OK;
     std::vector<HostOp> merged = merge_ops(ops);
@@ -571,7 +574,8 @@ static int forward_passes(DevicePlan &dp, dmv_state *s)
     const bool nt = dp.pp.nontemporal != 0;
     for (uint32_t p = 0; p < dp.plan.passes.size(); p++) {
         prof_begin(DMV_KC_TILE);
-        DMV_TRY(launch_tile(s->dtype, dp.pp.rlog, dp.pp.tlog, nt, s->amp, s->n, s->layout, dp.d_passes, dp.d_phases, dp.d_ops, p));
+        DMV_TRY(launch_tile(s->dtype, dp.pp.rlog, dp.pp.tlog, nt, dp.plan.passes[p].lds_half != 0, s->amp, s->n,
+                            s->layout, dp.d_passes, dp.d_phases, dp.d_ops, p));
         prof_end(DMV_KC_TILE, 2.0 * (double)s->bytes);
     }
     return DMV_OK;

@@ -37,11 +37,11 @@ template <int TLOG> __device__ __forceinline__ uint64_t thread_global_offset(con
 
 // Moves the tile through LDS: register p of thread t is written at
 // swz(thread part in A) ^ src[p] and register r is read back from
-// swz(thread part in B) ^ B->lswz[r].  The swizzle is linear, so these are
+// swz(thread part in B) ^ dst[r].  The swizzle is linear, so these are
 // the swizzled local indices of (t, p) in A and (t, r) in B.
 template <class V, int NR, int TLOG>
 __device__ __forceinline__ void transpose(V (&v)[NR], V *lds, const DevLayout *A, const uint32_t *src,
-                                          const DevLayout *B, uint32_t tid)
+                                          const DevLayout *B, const uint32_t *dst, uint32_t tid)
 {
     constexpr int G = sizeof(V) == 16 ? 3 : 4;
     char *base = reinterpret_cast<char *>(lds);
@@ -53,8 +53,45 @@ __device__ __forceinline__ void transpose(V (&v)[NR], V *lds, const DevLayout *A
     const uint32_t lb = swz_index<G>(thread_local_offset<TLOG>(B, tid)) * (uint32_t)sizeof(V);
 #pragma unroll
     for (int r = 0; r < NR; r++)
-        v[r] = *reinterpret_cast<const V *>(base + (lb ^ B->lswz[r]));
+        v[r] = *reinterpret_cast<const V *>(base + (lb ^ dst[r]));
     __syncthreads();
+}
+
+// The same through half a tile of LDS, in a pass whose layouts all hold one
+// local bit w at the top thread bit (a wave bit): the threads of w = 0
+// exchange their amplitudes first, then those of w = 1.  Offsets drop byte
+// bit `split` (w's bit, which the swizzle leaves in place); src and dst come
+// from the host with that bit already removed, and removing a bit is linear,
+// so drop(la ^ src[p]) = drop(la) ^ src[p].  Every wave passes the same four
+// barriers; the rounds are uniform branches on the wave's half.
+template <class V, int NR, int TLOG>
+__device__ __forceinline__ void transpose_half(V (&v)[NR], V *lds, const DevLayout *A, const uint32_t *src,
+                                               const DevLayout *B, const uint32_t *dst, uint32_t split,
+                                               uint32_t tid)
+{
+    static_assert(TLOG >= 7, "the split bit must be a wave bit");
+    constexpr int G = sizeof(V) == 16 ? 3 : 4;
+    char *base = reinterpret_cast<char *>(lds);
+    const uint32_t la =
+        drop_bit(swz_index<G>(thread_local_offset<TLOG>(A, tid)) * (uint32_t)sizeof(V), split);
+    const uint32_t lb =
+        drop_bit(swz_index<G>(thread_local_offset<TLOG>(B, tid)) * (uint32_t)sizeof(V), split);
+    const uint32_t half = __builtin_amdgcn_readfirstlane(tid >> (TLOG - 1));
+#pragma unroll
+    for (uint32_t h = 0; h < 2; h++) {
+        if (half == h) {
+#pragma unroll
+            for (int r = 0; r < NR; r++)
+                *reinterpret_cast<V *>(base + (la ^ src[r])) = v[r];
+        }
+        __syncthreads();
+        if (half == h) {
+#pragma unroll
+            for (int r = 0; r < NR; r++)
+                v[r] = *reinterpret_cast<const V *>(base + (lb ^ dst[r]));
+        }
+        __syncthreads();
+    }
 }
 
 constexpr int highest_bit(int m) { return m >= 16 ? 16 : m >= 8 ? 8 : m >= 4 ? 4 : m >= 2 ? 2 : 1; }
@@ -400,8 +437,11 @@ __device__ __forceinline__ void store_tile(const V (&v)[NR], V *__restrict__ sta
         st<NT>(state + (eg ^ greg[r]), v[r]);
 }
 
-// Runs the phases of a pass on a loaded tile and leaves it in the store layout.
-template <class R, int RLOG, int TLOG>
+// Runs the phases of a pass on a loaded tile and leaves it in the store
+// layout.  HALF: lds holds half a tile and every transpose of the pass is
+// split (the host runs only such passes, or passes without transposes, on
+// these kernels).
+template <class R, int RLOG, int TLOG, bool HALF>
 __device__ __forceinline__ void process_tile(typename Cplx<R>::T (&v)[1 << RLOG], const DevPass *P,
                                              const DevPhase *__restrict__ phases, const DevOp *__restrict__ ops,
                                              typename Cplx<R>::T *lds)
@@ -412,17 +452,25 @@ __device__ __forceinline__ void process_tile(typename Cplx<R>::T (&v)[1 << RLOG]
     const DevLayout *cur = &P->lay_ld;
     for (uint32_t ph = P->phase_begin; ph < P->phase_end; ph++) {
         const DevPhase *D = phases + ph;
-        if (D->transpose_in)
-            transpose<V, NR, TLOG>(v, lds, cur, D->src_lswz, &D->lay, tid);
+        if (D->transpose_in) {
+            if constexpr (HALF)
+                transpose_half<V, NR, TLOG>(v, lds, cur, D->src_lswz, &D->lay, D->dst_lswz, P->split, tid);
+            else
+                transpose<V, NR, TLOG>(v, lds, cur, D->src_lswz, &D->lay, D->dst_lswz, tid);
+        }
         cur = &D->lay;
         run_ops<V, R, RLOG>(v, ops, D->op_begin, D->op_end, tid);
     }
-    if (P->transpose_st)
-        transpose<V, NR, TLOG>(v, lds, cur, P->st_src_lswz, &P->lay_st, tid);
+    if (P->transpose_st) {
+        if constexpr (HALF)
+            transpose_half<V, NR, TLOG>(v, lds, cur, P->st_src_lswz, &P->lay_st, P->st_dst_lswz, P->split, tid);
+        else
+            transpose<V, NR, TLOG>(v, lds, cur, P->st_src_lswz, &P->lay_st, P->st_dst_lswz, tid);
+    }
 }
 
 // One tile of a forward pass: load, run the phases, store.
-template <class R, int RLOG, int TLOG, bool NT>
+template <class R, int RLOG, int TLOG, bool NT, bool HALF = false>
 __device__ __forceinline__ void tile_body(typename Cplx<R>::T *__restrict__ state, const DevPass *P,
                                           const DevPhase *__restrict__ phases, const DevOp *__restrict__ ops,
                                           uint64_t tile, typename Cplx<R>::T *lds, StateLayout L)
@@ -433,7 +481,7 @@ __device__ __forceinline__ void tile_body(typename Cplx<R>::T *__restrict__ stat
     const uint64_t g = tile_base<K>(P, tile);
     V v[NR];
     load_tile<NT>(v, state, g | thread_global_offset<TLOG>(P->gthr_ld, tid), P->greg_ld, L);
-    process_tile<R, RLOG, TLOG>(v, P, phases, ops, lds);
+    process_tile<R, RLOG, TLOG, HALF>(v, P, phases, ops, lds);
     store_tile<NT>(v, state, g | thread_global_offset<TLOG>(P->gthr_st, tid), P->greg_st, L);
 }
 
@@ -521,8 +569,8 @@ __device__ __forceinline__ void tile_body_adjoint(typename Cplx<R>::T *__restric
     for (uint32_t ph = P->phase_begin; ph < P->phase_end; ph++) {
         const DevPhase *D = phases + ph;
         if (D->transpose_in) {
-            transpose<V, NR, TLOG>(p, lds, cur, D->src_lswz, &D->lay, tid);
-            transpose<V, NR, TLOG>(l, lds, cur, D->src_lswz, &D->lay, tid);
+            transpose<V, NR, TLOG>(p, lds, cur, D->src_lswz, &D->lay, D->dst_lswz, tid);
+            transpose<V, NR, TLOG>(l, lds, cur, D->src_lswz, &D->lay, D->dst_lswz, tid);
         }
         cur = &D->lay;
         for (uint32_t o = D->op_begin; o < D->op_end; o++) {
@@ -558,8 +606,8 @@ __device__ __forceinline__ void tile_body_adjoint(typename Cplx<R>::T *__restric
         }
     }
     if (P->transpose_st) {
-        transpose<V, NR, TLOG>(p, lds, cur, P->st_src_lswz, &P->lay_st, tid);
-        transpose<V, NR, TLOG>(l, lds, cur, P->st_src_lswz, &P->lay_st, tid);
+        transpose<V, NR, TLOG>(p, lds, cur, P->st_src_lswz, &P->lay_st, P->st_dst_lswz, tid);
+        transpose<V, NR, TLOG>(l, lds, cur, P->st_src_lswz, &P->lay_st, P->st_dst_lswz, tid);
     }
     const uint64_t gs = g | thread_global_offset<TLOG>(P->gthr_st, tid);
     store_tile<false>(p, psi, gs, P->greg_st, L);

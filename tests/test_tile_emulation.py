@@ -210,3 +210,50 @@ def test_element_map_properties(shift, blk, fold):
         hi |= ((np.arange(512) >> j) & 1) << (23 + j + (shift - 17))
     low_span = (element_map(hi, shift, blk, fold) >> shift) & 63
     assert len(np.unique(low_span)) == (1 << fold if fold else 1)
+
+
+@pytest.mark.parametrize("dtype,k,elem", [(N.C128, 12, 16), (N.C64, 13, 8)])
+def test_split_transposes_emulation(dtype, k, elem):
+    """Half-LDS passes: the planner keeps one local bit at the top thread bit
+    of every layout of a pass, and each transpose runs as two rounds (the
+    threads of either half) through half a tile of LDS.  Emulated round by
+    round with the LDS cleared in between, against the oracle; with the
+    tunable off, the same circuit plans full transposes."""
+    n = 16
+    rng = np.random.default_rng(41)
+    gates = random_gates(rng, n, 300)
+    nc = NativeCircuit(n, gates)
+    plan = plan_json(nc.h, dtype=dtype, tile_qubits=k, tlog=8, max_gen=8)
+    split = [P for P in plan["passes"] if P["split"]]
+    assert len(split) >= len(plan["passes"]) // 2
+    assert any(P["transpose_st"] or any(ph["transpose_in"] for ph in P["phases"]) for P in split)
+    for P in split:
+        w = P["split"] - (4 if elem == 16 else 3)
+        lays = [P["lay_ld"], P["lay_st"]] + [ph["lay"] for ph in P["phases"]]
+        assert all(L["thr"][7] == w for L in lays)
+        assert P["lds_half"] == 1
+    got = TileEmulator(plan, elem=elem).forward(zero_state(n))
+    np.testing.assert_allclose(got, oracle.run_gates(n, gates).amp, atol=1e-12 if elem == 16 else 1e-12)
+    N.tune_set("half_lds", 0)
+    try:
+        full = plan_json(nc.h, dtype=dtype, tile_qubits=k, tlog=8, max_gen=8)
+    finally:
+        N.tune_set("half_lds", 1)
+    assert all(P["split"] == 0 and P["lds_half"] == 0 for P in full["passes"])
+    got = TileEmulator(full, elem=elem).forward(zero_state(n))
+    np.testing.assert_allclose(got, oracle.run_gates(n, gates).amp, atol=1e-12)
+
+
+def test_planted_split_mistake_is_caught():
+    """A split on a bit that is not the layouts' top thread bit moves amplitudes
+    between the halves: the emulated rounds then read what their round did
+    not write (or write outside half the LDS)."""
+    n = 15
+    rng = np.random.default_rng(43)
+    gates = random_gates(rng, n, 200)
+    nc = NativeCircuit(n, gates)
+    plan = plan_json(nc.h, tile_qubits=12, tlog=8, max_gen=8)
+    P = next(P for P in plan["passes"] if P["split"] and any(ph["transpose_in"] for ph in P["phases"]))
+    P["split"] -= 1
+    with pytest.raises(AssertionError):
+        TileEmulator(plan).forward(zero_state(n))

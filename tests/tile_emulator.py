@@ -36,6 +36,13 @@ def swz(li, G):
     return li ^ (f & ((1 << G) - 1))
 
 
+def drop_bit(y, b):
+    """An LDS byte offset with bit b removed, the bits above it shifted down."""
+    y = np.asarray(y, dtype=np.int64)
+    lo = (1 << b) - 1
+    return (y & lo) | ((y >> 1) & ~lo)
+
+
 def insert_zero(x, pos):
     low = x & ((1 << pos) - 1)
     return ((x >> pos) << (pos + 1)) | low
@@ -94,24 +101,37 @@ class TileEmulator:
             t = insert_zero(t, q)
         return t
 
-    def transpose(self, v, A, src, B):
-        """v: [tiles, NT, NR]; writes register p at swz(lt_A)^src[p], reads r at swz(lt_B)^lswz_B[r]."""
+    def transpose(self, v, A, src, B, dst, split=0):
+        """v: [tiles, NT, NR]; writes register p at swz(lt_A)^src[p], reads r at swz(lt_B)^dst[r].
+        split > 0: as the half-LDS kernels do it, two rounds (threads with top thread bit 0, then 1)
+        through half the LDS, byte offsets with bit `split` removed (tables already without it)."""
         tiles = v.shape[0]
-        lds = np.full((tiles, 1 << self.k), np.nan + 0j)
-        hits = np.zeros(1 << self.k, dtype=np.int64)
-        la = swz(self.thread_local(A), self.G)
-        for p in range(self.NR):
-            assert src[p] % self.elem == 0
-            addr = la ^ (src[p] // self.elem)
-            assert addr.max() < (1 << self.k)
-            lds[:, addr] = v[:, :, p]
-            np.add.at(hits, addr, 1)
-        assert np.all(hits == 1), "transpose writes are not a bijection"
-        lb = swz(self.thread_local(B), self.G)
-        out = np.empty_like(v)
-        for r in range(self.NR):
-            out[:, :, r] = lds[:, lb ^ (B["lswz"][r] // self.elem)]
-        assert not np.isnan(out.real).any()
+        la = swz(self.thread_local(A), self.G) * self.elem
+        lb = swz(self.thread_local(B), self.G) * self.elem
+        if split:
+            la, lb = drop_bit(la, split), drop_bit(lb, split)
+            rounds = [self.tids >> (self.tlog - 1) == h for h in (0, 1)]
+            size = 1 << (self.k - 1)
+        else:
+            rounds = [np.ones(self.NT, dtype=bool)]
+            size = 1 << self.k
+        out = np.full_like(v, np.nan)
+        for sel in rounds:
+            lds = np.full((tiles, size), np.nan + 0j)  # nothing survives from the other round
+            hits = np.zeros(size, dtype=np.int64)
+            for p in range(self.NR):
+                addr = la[sel] ^ src[p]
+                assert np.all(addr % self.elem == 0)
+                addr = addr // self.elem
+                assert addr.max() < size, "transpose write beyond the kernel's LDS"
+                lds[:, addr] = v[:, sel, p]
+                np.add.at(hits, addr, 1)
+            assert np.all(hits == 1), "transpose writes are not a bijection"
+            for r in range(self.NR):
+                addr = (lb[sel] ^ dst[r]) // self.elem
+                assert addr.max() < size
+                out[:, sel, r] = lds[:, addr]
+        assert not np.isnan(out.real).any(), "transpose read an amplitude its round did not write"
         return out
 
     def apply(self, v, op, lam=None, sink=None):
@@ -196,9 +216,9 @@ class TileEmulator:
         cur = P["lay_ld"]
         for ph in P["phases"]:
             if ph["transpose_in"]:
-                v = self.transpose(v, cur, ph["src_lswz"], ph["lay"])
+                v = self.transpose(v, cur, ph["src_lswz"], ph["lay"], ph["dst_lswz"], P["split"])
                 if lv is not None:
-                    lv = self.transpose(lv, cur, ph["src_lswz"], ph["lay"])
+                    lv = self.transpose(lv, cur, ph["src_lswz"], ph["lay"], ph["dst_lswz"], P["split"])
             cur = ph["lay"]
             for op in ph["ops"]:
                 if op["kind"] in (OP_IPG, OP_IPZ):
@@ -208,9 +228,9 @@ class TileEmulator:
                     if lv is not None:
                         self.apply(lv, op)
         if P["transpose_st"]:
-            v = self.transpose(v, cur, P["st_src_lswz"], P["lay_st"])
+            v = self.transpose(v, cur, P["st_src_lswz"], P["lay_st"], P["st_dst_lswz"], P["split"])
             if lv is not None:
-                lv = self.transpose(lv, cur, P["st_src_lswz"], P["lay_st"])
+                lv = self.transpose(lv, cur, P["st_src_lswz"], P["lay_st"], P["st_dst_lswz"], P["split"])
         gs = self.thread_global(P["gthr_st"])
         idx_st = self.element(g[:, None, None] | gs[None, :, None]) ^ np.array(P["greg_st"], dtype=np.int64)[None, None, :]
         flat = np.sort(idx_st.reshape(-1))
