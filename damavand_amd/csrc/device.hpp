@@ -78,6 +78,37 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
+// The same without LDS round trips: DPP moves inside each row of 16 lanes
+// (xor 1 and 2 by quad permutes, then the half-row and row mirrors, each
+// bringing the other half's sum once the halves are uniform), then the
+// gfx950 row swaps: permlane16_swap of a value with itself returns rows
+// (0,0,2,2) and (1,1,3,3), whose sum is the pair sum; permlane32_swap
+// returns halves (lo,lo) and (hi,hi).  Every lane ends with the same value
+// (each step adds the same two terms in every lane).
+template <int CTRL> __device__ __forceinline__ double dpp_mov(double v)
+{
+    int2 x = __builtin_bit_cast(int2, v);
+    x.x = __builtin_amdgcn_mov_dpp(x.x, CTRL, 0xF, 0xF, false);
+    x.y = __builtin_amdgcn_mov_dpp(x.y, CTRL, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, x);
+}
+
+__device__ __forceinline__ double wave_sum_rows(double v)
+{
+    v += dpp_mov<0xB1>(v);   // quad_perm [1,0,3,2]
+    v += dpp_mov<0x4E>(v);   // quad_perm [2,3,0,1]
+    v += dpp_mov<0x141>(v);  // row_half_mirror
+    v += dpp_mov<0x140>(v);  // row_mirror
+    int2 x = __builtin_bit_cast(int2, v);
+    const auto a = __builtin_amdgcn_permlane16_swap(x.x, x.x, false, false);
+    const auto b = __builtin_amdgcn_permlane16_swap(x.y, x.y, false, false);
+    v = __builtin_bit_cast(double, int2{(int)a[0], (int)b[0]}) + __builtin_bit_cast(double, int2{(int)a[1], (int)b[1]});
+    x = __builtin_bit_cast(int2, v);
+    const auto c = __builtin_amdgcn_permlane32_swap(x.x, x.x, false, false);
+    const auto d = __builtin_amdgcn_permlane32_swap(x.y, x.y, false, false);
+    return __builtin_bit_cast(double, int2{(int)c[0], (int)d[0]}) + __builtin_bit_cast(double, int2{(int)c[1], (int)d[1]});
+}
+
 // Sum over a workgroup of NT threads; `scratch` holds NT/64 doubles.  The
 // result is valid in every thread.  Fixed order, so deterministic.
 template <int NT> __device__ __forceinline__ double block_sum(double v, double *scratch)

@@ -872,6 +872,12 @@ int validate_plan(const Plan &plan, const PlanParams &pp)
                     set_error("internal: op masks out of range");
                     return DMV_ERR_STATE;
                 }
+                // GENs come from uncontrolled gates only; the reverse sweep's
+                // kernels apply them to every pair of every lane
+                if (op.kind == OP_GEN && (op.ctrl != 0 || op.ctrl_thr != 0 || op.en != regs)) {
+                    set_error("internal: controlled GEN op");
+                    return DMV_ERR_STATE;
+                }
                 if (op_is_ip(op.kind)) {
                     const DevOp *prev = o > D.op_begin ? &plan.ops[o - 1] : nullptr;
                     if (op.plain && !(prev && prev->kind == op.kind && prev->target == op.target &&

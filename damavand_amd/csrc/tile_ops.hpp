@@ -96,10 +96,14 @@ __device__ __forceinline__ void transpose_half(V (&v)[NR], V *lds, const DevLayo
 
 constexpr int highest_bit(int m) { return m >= 16 ? 16 : m >= 8 ? 8 : m >= 4 ? 4 : m >= 2 ? 2 : 1; }
 
-// Pair op on registers (p, p ^ M).  f: p holds the target-bit-1 member, so
-// the matrix is applied with rows and columns exchanged.
+// In-place pair ops of the reverse sweep.  A GEN on registers (p, p ^ M)
+// of psi and lambda at once, so the per-pair coefficients (scalar selects)
+// are chosen once for both states.  f: p holds the target-bit-1 member, so
+// the matrix is applied with rows and columns exchanged.  GEN ops carry no
+// controls (plans only make them from uncontrolled gates, and plan
+// validation checks it), so no pair or lane is left out.
 template <int M, class V, class R, int NR>
-__device__ __forceinline__ void op_gen_m(V (&v)[NR], const Mat<R> &A, uint32_t en, uint32_t fl)
+__device__ __forceinline__ void op_gen2_m(V (&a)[NR], V (&b)[NR], const Mat<R> &A, uint32_t fl)
 {
     constexpr int HB = highest_bit(M);
 #pragma unroll
@@ -112,22 +116,26 @@ __device__ __forceinline__ void op_gen_m(V (&v)[NR], const Mat<R> &A, uint32_t e
         B.m01r = f ? A.m10r : A.m01r; B.m01i = f ? A.m10i : A.m01i;
         B.m10r = f ? A.m01r : A.m10r; B.m10i = f ? A.m01i : A.m10i;
         B.m11r = f ? A.m00r : A.m11r; B.m11i = f ? A.m00i : A.m11i;
-        V x = v[p], y = v[p ^ M];
-        apply2(x, y, B);
-        const bool ok = (en >> p) & 1u;
-        v[p] = ok ? x : v[p];
-        v[p ^ M] = ok ? y : v[p ^ M];
+        apply2(a[p], a[p ^ M], B);
+        __builtin_amdgcn_sched_barrier(0);
+        apply2(b[p], b[p ^ M], B);
+        __builtin_amdgcn_sched_barrier(0);
     }
 }
 
-template <int M, class V, int NR> __device__ __forceinline__ void op_swap_m(V (&v)[NR], uint32_t en)
+// Swaps and diagonals act where the pair's register controls (en) and the
+// lane's thread controls (hold) pass; both enter the selects they make
+// anyway, so that no lane branches around an op (a divergent branch around
+// in-place updates makes the compiler copy every register of both states
+// before each op).
+template <int M, class V, int NR> __device__ __forceinline__ void op_swap_m(V (&v)[NR], uint32_t en, bool hold)
 {
     constexpr int HB = highest_bit(M);
 #pragma unroll
     for (int p = 0; p < NR; p++) {
         if (p & HB)
             continue;
-        const bool ok = (en >> p) & 1u;
+        const bool ok = ((en >> p) & 1u) && hold;
         const V x = v[p], y = v[p ^ M];
         v[p] = ok ? y : x;
         v[p ^ M] = ok ? x : y;
@@ -144,40 +152,46 @@ template <int M, class V, int NR> __device__ __forceinline__ void op_swap_m(V (&
     X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17)        \
     X(18) X(19) X(20) X(21) X(22) X(23) X(24) X(25) X(26) X(27) X(28) X(29) X(30) X(31)
 
+// A GEN on psi and lambda (one dispatch, shared coefficient selects).
 template <class V, class R, int NR>
-__device__ __forceinline__ void op_pair(V (&v)[NR], const DevOp *op)
+__device__ __forceinline__ void op_gen2(V (&a)[NR], V (&b)[NR], const DevOp *op)
 {
-    const uint32_t en = op->en, fl = op->fl;
-    if (op->kind == OP_GEN) {
-        const Mat<R> A = load_mat<R>(op->m);
-        switch (op->rmask) {
-#define DMV_GEN_CASE(M)                                                                            \
+    const uint32_t fl = op->fl;
+    const Mat<R> A = load_mat<R>(op->m);
+    switch (op->rmask) {
+#define DMV_GEN2_CASE(M)                                                                           \
     case M:                                                                                        \
         if constexpr (M < NR)                                                                      \
-            op_gen_m<M>(v, A, en, fl);                                                             \
+            op_gen2_m<M>(a, b, A, fl);                                                             \
         break;
-            DMV_MASK_CASES(DMV_GEN_CASE)
-#undef DMV_GEN_CASE
-        default: break;
-        }
-    } else {
-        switch (op->rmask) {
+        DMV_MASK_CASES(DMV_GEN2_CASE)
+#undef DMV_GEN2_CASE
+    default: break;
+    }
+}
+
+// SWAP (X / CNOT that renaming could not absorb) on registers (p, p ^ rmask).
+template <class V, int NR>
+__device__ __forceinline__ void op_swap(V (&v)[NR], const DevOp *op, bool hold)
+{
+    const uint32_t en = op->en;
+    switch (op->rmask) {
 #define DMV_SWAP_CASE(M)                                                                           \
     case M:                                                                                        \
         if constexpr (M < NR)                                                                      \
-            op_swap_m<M>(v, en);                                                                   \
+            op_swap_m<M>(v, en, hold);                                                             \
         break;
-            DMV_MASK_CASES(DMV_SWAP_CASE)
+        DMV_MASK_CASES(DMV_SWAP_CASE)
 #undef DMV_SWAP_CASE
-        default: break;
-        }
+    default: break;
     }
 }
 
 // diag(m00, m11) on a register-bit target (rmask: registers with target bit
-// 1) or on a thread-bit target (tbit), on the registers in en.
+// 1) or on a thread-bit target (tbit), on the registers in en, in lanes
+// whose thread controls hold.
 template <class V, class R, int NR>
-__device__ __forceinline__ void op_diag(V (&v)[NR], const DevOp *op, uint32_t tid)
+__device__ __forceinline__ void op_diag(V (&v)[NR], const DevOp *op, uint32_t tid, bool hold)
 {
     const R d0r = (R)op->m[0], d0i = (R)op->m[1], d1r = (R)op->m[6], d1i = (R)op->m[7];
     const uint32_t en = op->en;
@@ -188,7 +202,7 @@ __device__ __forceinline__ void op_diag(V (&v)[NR], const DevOp *op, uint32_t ti
         for (int r = 0; r < NR; r++) {
             const bool one = (ones >> r) & 1u;
             const V x = cmul(v[r], one ? d1r : d0r, one ? d1i : d0i);
-            v[r] = (en == all || ((en >> r) & 1u)) ? x : v[r];
+            v[r] = ((en == all || ((en >> r) & 1u)) && hold) ? x : v[r];
         }
     } else {
         const bool one = (tid >> op->tbit) & 1u;
@@ -196,7 +210,7 @@ __device__ __forceinline__ void op_diag(V (&v)[NR], const DevOp *op, uint32_t ti
 #pragma unroll
         for (int r = 0; r < NR; r++) {
             const V x = cmul(v[r], dr, di);
-            v[r] = (en == all || ((en >> r) & 1u)) ? x : v[r];
+            v[r] = ((en == all || ((en >> r) & 1u)) && hold) ? x : v[r];
         }
     }
 }
@@ -273,9 +287,10 @@ __device__ __forceinline__ void oop_gen_plain_m(V (&out)[NR], const V (&in)[NR],
     }
 }
 
+// A GEN whose pair members are swapped where fl says so (GENs carry no
+// controls, so every pair of every lane takes it).
 template <int M, class V, class R, int NR>
-__device__ __forceinline__ void oop_gen_m(V (&out)[NR], const V (&in)[NR], const Mat<R> &A, uint32_t en, uint32_t fl,
-                                          bool hold)
+__device__ __forceinline__ void oop_gen_m(V (&out)[NR], const V (&in)[NR], const Mat<R> &A, uint32_t fl)
 {
     constexpr int HB = highest_bit(M);
 #pragma unroll
@@ -288,11 +303,7 @@ __device__ __forceinline__ void oop_gen_m(V (&out)[NR], const V (&in)[NR], const
         B.m01r = f ? A.m10r : A.m01r; B.m01i = f ? A.m10i : A.m01i;
         B.m10r = f ? A.m01r : A.m10r; B.m10i = f ? A.m01i : A.m10i;
         B.m11r = f ? A.m00r : A.m11r; B.m11i = f ? A.m00i : A.m11i;
-        V x, y;
-        apply2_to(x, y, in[p], in[p ^ M], B);
-        const bool ok = ((en >> p) & 1u) && hold;
-        out[p] = ok ? x : in[p];
-        out[p ^ M] = ok ? y : in[p ^ M];
+        apply2_to(out[p], out[p ^ M], in[p], in[p ^ M], B);
     }
 }
 
@@ -336,8 +347,8 @@ __device__ __forceinline__ void oop_diag(V (&out)[NR], const V (&in)[NR], const 
 }
 
 // One op from `in` into `out`; lanes whose thread-bit controls do not hold
-// keep their amplitudes: through the selects the op makes anyway, or by a
-// copy after a plain GEN, which makes none.
+// keep their amplitudes through the selects that swaps and diagonals make
+// anyway (GENs carry no controls).
 template <class V, class R, int NR>
 __device__ __forceinline__ void oop_op(V (&out)[NR], const V (&in)[NR], const DevOp *op, uint32_t tid)
 {
@@ -357,8 +368,6 @@ __device__ __forceinline__ void oop_op(V (&out)[NR], const V (&in)[NR], const De
 #undef DMV_OOP_PLAIN_CASE
         default: oop_gen_plain_m<1>(out, in, A); break;
         }
-        if (op->ctrl_thr && !hold)
-            copy_regs(out, in);
     } else if (op->kind == OP_DIAG) {
         oop_diag<V, R>(out, in, op, tid, hold);
     } else if (op->kind == OP_GEN) {
@@ -366,11 +375,11 @@ __device__ __forceinline__ void oop_op(V (&out)[NR], const V (&in)[NR], const De
 #define DMV_OOP_GEN_CASE(M)                                                                        \
     case M:                                                                                        \
         if constexpr (M < NR)                                                                      \
-            oop_gen_m<M>(out, in, A, en, fl, hold);                                                \
+            oop_gen_m<M>(out, in, A, fl);                                                          \
         break;
             DMV_MASK_CASES_FROM2(DMV_OOP_GEN_CASE)
 #undef DMV_OOP_GEN_CASE
-        default: oop_gen_m<1>(out, in, A, en, fl, hold); break;
+        default: oop_gen_m<1>(out, in, A, fl); break;
         }
     } else {
         switch (m) {
@@ -501,13 +510,23 @@ __device__ __forceinline__ void ip_pair_m(const V (&p)[NR], const V (&l)[NR], ui
     for (int a = 0; a < NR; a++) {
         if (a & HB)
             continue;
-        const V pa = p[a], pb = p[a ^ M], la = l[a], lb = l[a ^ M];
-        sx += (double)la.x * pb.y - (double)la.y * pb.x + (double)lb.x * pa.y - (double)lb.y * pa.x;
-        const double y = -((double)la.x * pb.x + (double)la.y * pb.y) + ((double)lb.x * pa.x + (double)lb.y * pa.y);
-        const double z = ((double)la.x * pa.y - (double)la.y * pa.x) - ((double)lb.x * pb.y - (double)lb.y * pb.x);
-        const bool f = (fl >> a) & 1u;
-        sy += f ? -y : y;
-        sz += f ? -z : z;
+        const double pax = p[a].x, pay = p[a].y, pbx = p[a ^ M].x, pby = p[a ^ M].y;
+        const double lax = l[a].x, lay = l[a].y, lbx = l[a ^ M].x, lby = l[a ^ M].y;
+        sx = __builtin_fma(lax, pby, sx);
+        sx = __builtin_fma(-lay, pbx, sx);
+        sx = __builtin_fma(lbx, pay, sx);
+        sx = __builtin_fma(-lby, pax, sx);
+        double y = lbx * pax;
+        y = __builtin_fma(lby, pay, y);
+        y = __builtin_fma(-lax, pbx, y);
+        y = __builtin_fma(-lay, pby, y);
+        double z = lax * pay;
+        z = __builtin_fma(-lay, pax, z);
+        z = __builtin_fma(-lbx, pby, z);
+        z = __builtin_fma(lby, pbx, z);
+        const double sg = ((fl >> a) & 1u) ? -1.0 : 1.0;  // a scalar: fl is uniform
+        sy = __builtin_fma(sg, y, sy);
+        sz = __builtin_fma(sg, z, sz);
     }
 }
 
@@ -526,8 +545,9 @@ __device__ __forceinline__ void ip_sums(const V (&p)[1 << RLOG], const V (&l)[1 
 #pragma unroll
         for (int r = 0; r < NR; r++) {
             const bool one = reg ? ((ones >> r) & 1u) : thread_one;
-            const double w = (double)l[r].x * p[r].y - (double)l[r].y * p[r].x;
-            sz += one ? -w : w;
+            double w = (double)l[r].x * p[r].y;
+            w = __builtin_fma(-(double)l[r].y, p[r].x, w);
+            sz = __builtin_fma(one ? -1.0 : 1.0, w, sz);
         }
         return;
     }
@@ -580,8 +600,8 @@ __device__ __forceinline__ void tile_body_adjoint(typename Cplx<R>::T *__restric
                     ip_sums<V, RLOG>(p, l, op, tid, sx, sy, sz);
                 const double v = op->m[0] * sx + op->m[1] * sy + op->m[2] * sz;
                 if constexpr (Sink::kPerWave) {
-                    // one partial per wave: shuffles only, no workgroup barrier
-                    const double w = wave_sum(v);
+                    // one partial per wave: lane moves only, no workgroup barrier
+                    const double w = wave_sum_rows(v);
                     if ((tid & 63) == 0)
                         sink(op, w, tid >> 6);
                 } else {
@@ -591,17 +611,18 @@ __device__ __forceinline__ void tile_body_adjoint(typename Cplx<R>::T *__restric
                 }
                 continue;
             }
-            if (!thread_controls_hold(op, tid))
-                continue;
+            const bool hold = thread_controls_hold(op, tid);
             if (op->plain) {
                 op_gen_plain<V, R>(p, op);
                 op_gen_plain<V, R>(l, op);
+            } else if (op->kind == OP_GEN) {
+                op_gen2<V, R>(p, l, op);
             } else if (op->kind == OP_DIAG) {
-                op_diag<V, R>(p, op, tid);
-                op_diag<V, R>(l, op, tid);
+                op_diag<V, R>(p, op, tid, hold);
+                op_diag<V, R>(l, op, tid, hold);
             } else {
-                op_pair<V, R>(p, op);
-                op_pair<V, R>(l, op);
+                op_swap<V>(p, op, hold);
+                op_swap<V>(l, op, hold);
             }
         }
     }
