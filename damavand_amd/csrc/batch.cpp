@@ -23,11 +23,12 @@ namespace dmv {
 // threads, two workgroups per CU.
 static void batch_shape(int nmax, int dtype, const TuneChoice *pin, int *k, int *tlog, int *wgs)
 {
-    // measured defaults on the 4096-circuit 12-18 qubit batch: complex64
-    // runs 200 ms at two workgroups per CU and 159 ms at three
+    // three workgroups per CU (the kernel keeps to 168 VGPRs for it);
+    // measured on the 4096-circuit 12-18 qubit batch: complex128 219 -> 183-193
+    // ms and complex64 200 -> 159 ms against two
     *k = 11;
     *tlog = 8;
-    *wgs = dtype == DMV_C64 ? 3 : 2;
+    *wgs = 3;
     TuneChoice t;
     if (tune_lookup(tune_key(nmax, dtype, "batch"), &t) && t.set()) {
         *k = t.k;

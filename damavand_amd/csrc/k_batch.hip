@@ -33,8 +33,9 @@ __device__ __forceinline__ double chain_h(uint64_t x, uint32_t n, const double *
     return h;
 }
 
+// At most 168 VGPRs, so that three 256-thread workgroups share a CU.
 template <class R, int RLOG, int TLOG>
-__global__ void __launch_bounds__(1 << TLOG)
+__global__ void __launch_bounds__(1 << TLOG) __attribute__((amdgpu_waves_per_eu(3)))
 k_batch(const BatchCircuit *__restrict__ circ, const uint32_t *__restrict__ order, uint32_t count,
         uint32_t *__restrict__ counter, const DevPass *__restrict__ passes, const DevPhase *__restrict__ phases,
         const DevOp *__restrict__ ops, const DevOp *__restrict__ sops, const double *__restrict__ coef,
