@@ -83,7 +83,9 @@ constexpr uint32_t kNoReg = 0xFFFFFFFFu;
 //  * ctrl_thr: controls that are thread bits (mask over the thread id);
 //  * plain: a GEN op acting on every register (en = all) whose first pair
 //    members (registers with the partner mask's highest bit clear) all hold
-//    target bit 0, so the matrix applies without per-pair selects; for an IP
+//    target bit 0, so the matrix applies without per-pair selects; for a
+//    DIAG op: the diagonal is exactly diag(1, -1) (Z, CZ), so the op only
+//    flips the signs of the amplitudes whose target bit is 1; for an IP
 //    op: the previous op of the phase is an IP op of the same kind, target
 //    and masks, so its per-thread sums of X, Y and Z are reused.
 struct alignas(16) DevOp {
@@ -161,6 +163,13 @@ struct alignas(16) DevPass {
     uint32_t lds_half;  // 1: the pass fits kernels with half a tile of LDS
     uint32_t pad[2];
 };
+
+// A diagonal the kernels apply as sign flips: exactly diag(1, -1).
+inline bool diag_is_sign_flip(const double *m)
+{
+    return m[0] == 1.0 && m[1] == 0.0 && m[2] == 0.0 && m[3] == 0.0 && m[4] == 0.0 && m[5] == 0.0 &&
+           m[6] == -1.0 && m[7] == 0.0;
+}
 
 // An LDS byte offset with bit b removed (the bits above it shift down).
 inline DMV_HD uint32_t drop_bit(uint32_t y, uint32_t b)

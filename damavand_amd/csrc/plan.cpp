@@ -623,6 +623,8 @@ static int make_pass(uint64_t Q, const std::vector<HostOp> &taken, const PlanPar
             }
             if (h.kind == OP_GEN && J != kNoReg)
                 op.plain = (op.en == all_regs && (op.fl & first_members(op.rmask, NR)) == 0) ? 1u : 0u;
+            if (h.kind == OP_DIAG)
+                op.plain = diag_is_sign_flip(op.m) ? 1u : 0u;
             if (op_is_ip(h.kind) && h.slot + 1 > plan->n_slots)
                 plan->n_slots = h.slot + 1;
             if (op_is_ip(h.kind) && plan->ops.size() > D.op_begin) {
@@ -883,6 +885,11 @@ int validate_plan(const Plan &plan, const PlanParams &pp)
                     if (op.plain && !(prev && prev->kind == op.kind && prev->target == op.target &&
                                       prev->tbit == op.tbit && prev->rmask == op.rmask && prev->fl == op.fl)) {
                         set_error("internal: IP op reuses sums it has no predecessor for");
+                        return DMV_ERR_STATE;
+                    }
+                } else if (op.plain && op.kind == OP_DIAG) {
+                    if (!diag_is_sign_flip(op.m)) {
+                        set_error("internal: plain flag on a diagonal other than diag(1, -1)");
                         return DMV_ERR_STATE;
                     }
                 } else if (op.plain &&

@@ -187,12 +187,58 @@ __device__ __forceinline__ void op_swap(V (&v)[NR], const DevOp *op, bool hold)
     }
 }
 
+// v with the signs of both parts flipped where m is the sign bit (m = 0
+// leaves it as it is): -v exactly, by integer XOR.
+template <class V> __device__ __forceinline__ V flip_sign(V v, uint32_t m)
+{
+    if constexpr (sizeof(V) == 16) {
+        using U = uint32_t __attribute__((ext_vector_type(4)));
+        U u = __builtin_bit_cast(U, v);
+        u.y ^= m;
+        u.w ^= m;
+        return __builtin_bit_cast(V, u);
+    } else {
+        using U = uint32_t __attribute__((ext_vector_type(2)));
+        U u = __builtin_bit_cast(U, v);
+        u.x ^= m;
+        u.y ^= m;
+        return __builtin_bit_cast(V, u);
+    }
+}
+
+// The sign mask of register r for a diag(1, -1) op (plain DIAG): the sign bit
+// where the register holds target bit 1 and the register and thread
+// controls hold.  lane: the sign bit in lanes whose thread controls hold
+// (and, for a thread-bit target, whose target bit is 1).
+template <int NR>
+__device__ __forceinline__ uint32_t flip_mask(const DevOp *op, int r, uint32_t lane)
+{
+    const uint32_t all = NR >= 32 ? 0xFFFFFFFFu : ((1u << NR) - 1);
+    const bool en_r = op->en == all || ((op->en >> r) & 1u);
+    const bool one_r = op->tbit != kNoReg || ((op->rmask >> r) & 1u);
+    return en_r && one_r ? lane : 0u;
+}
+
+__device__ __forceinline__ uint32_t flip_lane(const DevOp *op, uint32_t tid, bool hold)
+{
+    const bool one = op->tbit == kNoReg || ((tid >> op->tbit) & 1u);
+    return hold && one ? 0x80000000u : 0u;
+}
+
 // diag(m00, m11) on a register-bit target (rmask: registers with target bit
 // 1) or on a thread-bit target (tbit), on the registers in en, in lanes
-// whose thread controls hold.
+// whose thread controls hold.  diag(1, -1) (plain) flips signs instead of
+// multiplying.
 template <class V, class R, int NR>
 __device__ __forceinline__ void op_diag(V (&v)[NR], const DevOp *op, uint32_t tid, bool hold)
 {
+    if (op->plain) {
+        const uint32_t lane = flip_lane(op, tid, hold);
+#pragma unroll
+        for (int r = 0; r < NR; r++)
+            v[r] = flip_sign(v[r], flip_mask<NR>(op, r, lane));
+        return;
+    }
     const R d0r = (R)op->m[0], d0i = (R)op->m[1], d1r = (R)op->m[6], d1i = (R)op->m[7];
     const uint32_t en = op->en;
     const uint32_t all = NR >= 32 ? 0xFFFFFFFFu : ((1u << NR) - 1);
@@ -324,6 +370,13 @@ __device__ __forceinline__ void oop_swap_m(V (&out)[NR], const V (&in)[NR], uint
 template <class V, class R, int NR>
 __device__ __forceinline__ void oop_diag(V (&out)[NR], const V (&in)[NR], const DevOp *op, uint32_t tid, bool hold)
 {
+    if (op->plain) {  // diag(1, -1): sign flips
+        const uint32_t lane = flip_lane(op, tid, hold);
+#pragma unroll
+        for (int r = 0; r < NR; r++)
+            out[r] = flip_sign(in[r], flip_mask<NR>(op, r, lane));
+        return;
+    }
     const R d0r = (R)op->m[0], d0i = (R)op->m[1], d1r = (R)op->m[6], d1i = (R)op->m[7];
     const uint32_t en = op->en;
     const uint32_t all = NR >= 32 ? 0xFFFFFFFFu : ((1u << NR) - 1);
@@ -357,7 +410,9 @@ __device__ __forceinline__ void oop_op(V (&out)[NR], const V (&in)[NR], const De
     const Mat<R> A = load_mat<R>(op->m);
     // masks at or above NR have empty cases; the clamp lets the compiler drop them
     const uint32_t m = op->rmask < (uint32_t)NR ? op->rmask : 1u;
-    if (op->plain) {
+    if (op->kind == OP_DIAG) {
+        oop_diag<V, R>(out, in, op, tid, hold);
+    } else if (op->plain) {
         switch (m) {
 #define DMV_OOP_PLAIN_CASE(M)                                                                      \
     case M:                                                                                        \
@@ -368,8 +423,6 @@ __device__ __forceinline__ void oop_op(V (&out)[NR], const V (&in)[NR], const De
 #undef DMV_OOP_PLAIN_CASE
         default: oop_gen_plain_m<1>(out, in, A); break;
         }
-    } else if (op->kind == OP_DIAG) {
-        oop_diag<V, R>(out, in, op, tid, hold);
     } else if (op->kind == OP_GEN) {
         switch (m) {
 #define DMV_OOP_GEN_CASE(M)                                                                        \
@@ -612,14 +665,14 @@ __device__ __forceinline__ void tile_body_adjoint(typename Cplx<R>::T *__restric
                 continue;
             }
             const bool hold = thread_controls_hold(op, tid);
-            if (op->plain) {
+            if (op->kind == OP_DIAG) {
+                op_diag<V, R>(p, op, tid, hold);
+                op_diag<V, R>(l, op, tid, hold);
+            } else if (op->plain) {
                 op_gen_plain<V, R>(p, op);
                 op_gen_plain<V, R>(l, op);
             } else if (op->kind == OP_GEN) {
                 op_gen2<V, R>(p, l, op);
-            } else if (op->kind == OP_DIAG) {
-                op_diag<V, R>(p, op, tid, hold);
-                op_diag<V, R>(l, op, tid, hold);
             } else {
                 op_swap<V>(p, op, hold);
                 op_swap<V>(l, op, hold);

@@ -257,3 +257,27 @@ def test_planted_split_mistake_is_caught():
     P["split"] -= 1
     with pytest.raises(AssertionError):
         TileEmulator(plan).forward(zero_state(n))
+
+
+def test_sign_flip_diagonals():
+    """Z and CZ (exactly diag(1, -1)) are flagged plain and applied as sign
+    flips; S and T keep the general diagonal; register and thread-bit
+    targets and controls, against the oracle."""
+    n = 14
+    rng = np.random.default_rng(47)
+    gates = [(wl.GATE_H, -1, q, ()) for q in range(n)]
+    for _ in range(60):
+        t = int(rng.integers(n))
+        c = int(rng.integers(n - 1))
+        c += c >= t
+        k = [wl.GATE_CZ, wl.GATE_Z, wl.GATE_S, wl.GATE_T, wl.GATE_RY][int(rng.integers(5))]
+        gates.append((k, c if k == wl.GATE_CZ else -1, t, (float(rng.uniform(0, 6.3)),)))
+    nc = NativeCircuit(n, gates)
+    plan = plan_json(nc.h, tile_qubits=12, tlog=8, max_gen=8)
+    diags = [op for P in plan["passes"] for ph in P["phases"] for op in ph["ops"] if op["kind"] == 1]
+    flips = [op for op in diags if op["plain"]]
+    assert flips and len(flips) < len(diags)
+    assert all(op["m"] == [1.0, 0.0, 0.0, 0.0, 0.0, 0.0, -1.0, 0.0] for op in flips)
+    assert any(op["tbit"] < 64 for op in flips) and any(op["ctrl_thr"] for op in flips)
+    got = TileEmulator(plan).forward(zero_state(n))
+    np.testing.assert_allclose(got, oracle.run_gates(n, gates).amp, atol=1e-12)

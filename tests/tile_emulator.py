@@ -169,6 +169,9 @@ class TileEmulator:
             sink(op["slot"], acc)
             return
         if kind == OP_DIAG:
+            if op["plain"]:
+                # the kernels flip signs instead of multiplying: valid for diag(1, -1) only
+                assert list(m) == [1.0, 0.0, 0.0, 0.0, 0.0, 0.0, -1.0, 0.0], "plain flag on a general diagonal"
             for r in range(self.NR):
                 if not (en >> r) & 1:
                     continue
