@@ -338,12 +338,14 @@ int dmv_synchronize(void);
 
 /* ---- tuning ---- */
 /* Sets a tunable, e.g. "tile_qubits" / "adjoint_tile_qubits" (9..13),
- * "tile_threads_log2" (6..9), "max_gen_per_pass", "nontemporal" (0/1),
+ * "tile_threads_log2" (6..9), "max_gen_per_pass", "nontemporal" (0/1; default
+ * 1 for forward passes over states of 2 GiB and more),
  * "gate_pairs_per_thread" (1, 2, 4), "batch_tile_qubits",
  * "batch_threads_log2", "batch_wgs_per_cu", "graph" (0/1: replay forward
  * passes from a captured hipGraph), "half_lds" (0/1, default 1: forward
  * passes of the default tile shapes split their LDS transposes in two
  * rounds through half a tile of LDS, so three workgroups share a CU).
+ * A negative value unsets the tunable, so its default applies again;
  * dmv_tune_get returns -1 when unset. */
 int dmv_tune_set(const char *name, int64_t value);
 int dmv_tune_get(const char *name, int64_t *value);

@@ -94,8 +94,8 @@ def test_pipelined_tile_passes(native, n, grid, nt):
     try:
         c.forward()
     finally:
-        N.tune_set("tile_grid", 0)
-        N.tune_set("nontemporal", 0)
+        N.tune_set("tile_grid", -1)
+        N.tune_set("nontemporal", -1)
     err = np.max(np.abs(c.retrieve_amplitudes_on_host() - oracle.run_gates(n, gates).amp))
     say(f"  max-abs error {err:.3e}")
     assert err <= TOL
@@ -118,15 +118,13 @@ def test_tile_shapes(native, n, k, tlog, gen, extra):
     gates = random_gates(rng, n, 300)
     c = build_circuit(n, gates)
     knobs = dict(tile_qubits=k, tile_threads_log2=tlog, max_gen_per_pass=gen, **extra)
-    defaults = dict(tile_qubits=0, tile_threads_log2=0, max_gen_per_pass=8, tile_xcd_log2=3, coalesce_qubits=0,
-                    register_renaming=1)
     for key, v in knobs.items():
         N.tune_set(key, v)
     try:
         c.forward()
     finally:
         for key in knobs:
-            N.tune_set(key, defaults[key])
+            N.tune_set(key, -1)  # unset: the default applies again
     err = np.max(np.abs(c.retrieve_amplitudes_on_host() - oracle.run_gates(n, gates).amp))
     say(f"  passes={c.plan_stats()[0]} max-abs error {err:.3e}")
     assert err <= TOL
@@ -409,7 +407,7 @@ def test_hipgraph_replays(native):
             assert np.max(np.abs(got - want)) <= TOL, rep
             c.state.norm_sqr()  # eager work between replays
     finally:
-        N.tune_set("graph", 0)
+        N.tune_set("graph", -1)
 
 
 def _read_device(ptr, nbytes):
@@ -432,7 +430,7 @@ def test_states_of_other_layouts(native):
     try:
         folded = StateVector(n)
     finally:
-        N.tune_set("page_fold_span_bits", 15)
+        N.tune_set("page_fold_span_bits", -1)
     plain = StateVector(n)
     gates = [(wl.GATE_H, -1, q, ()) for q in range(n)] + [(wl.GATE_CNOT, 3, n - 1, ()),
                                                          (wl.GATE_RY, -1, n - 2, (0.4,)), (wl.GATE_CZ, n - 1, 1, ())]
@@ -469,8 +467,8 @@ def test_state_layout_swizzled(native, dtype, n, spread, fold):
     try:
         _check_state_layout(dtype, n, spread, fold)
     finally:
-        N.tune_set("spread_min_bytes", 2 << 30)
-        N.tune_set("page_fold_span_bits", 15)
+        N.tune_set("spread_min_bytes", -1)
+        N.tune_set("page_fold_span_bits", -1)
 
 
 def _check_state_layout(dtype, n, spread, fold):

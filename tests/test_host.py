@@ -1,6 +1,7 @@
 """CPU-only checks of libdamavand-gpu: it loads, exports every declared
 symbol, and its host logic (gate list, fused-pass scheduler, argument
 checking) is right.  No kernel is launched here."""
+import ctypes
 import math
 
 import numpy as np
@@ -320,3 +321,26 @@ def test_tune_cache_is_read_back_with_precedence(tmp_path):
     assert got["21/0"] == [12, 8, 8, 0]           # no entry: the defaults
     assert got["21/1"] == [11, 8, 8, 0]
     assert got["20/0/tunable"][0] == 10           # dmv_tune_set beats the table
+
+
+def test_tunable_unset_restores_default():
+    """A negative value unsets a tunable: dmv_tune_get reports -1 again and
+    plans take the default (here the forward tile shape of a 20-qubit
+    circuit)."""
+    from tests.libhelpers import NativeCircuit
+    c = NativeCircuit(20, wl.layered_circuit(20, 1, 5))
+
+    def shape():
+        v = [ctypes.c_int(0) for _ in range(4)]
+        N.call("dmv_circuit_tile_shape", c.h, N.C128, 0, *[ctypes.byref(x) for x in v])
+        return [x.value for x in v]
+
+    default = shape()
+    N.tune_set("tile_qubits", 10)
+    try:
+        assert N.tune_get("tile_qubits") == 10
+        assert shape()[0] == 10
+    finally:
+        N.tune_set("tile_qubits", -1)
+    assert N.tune_get("tile_qubits") == -1
+    assert shape() == default

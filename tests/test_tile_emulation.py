@@ -164,7 +164,7 @@ def test_page_fold_emulation():
     try:
         plan = plan_json(nc.h, tile_qubits=12, tlog=8, max_gen=8)
     finally:
-        N.tune_set("page_fold_span_bits", 15)
+        N.tune_set("page_fold_span_bits", -1)
     assert plan["layout"] == [17, 6, 6]
     # some register offset has a span part of 64 or more, so the fold acts on it
     assert any((g >> 17) >= 64 for P in plan["passes"] for g in P["greg_ld"])
@@ -183,7 +183,7 @@ def test_wider_coalescing_runs_emulation(runs):
     try:
         plan = plan_json(nc.h, tile_qubits=12, tlog=8, max_gen=8)
     finally:
-        N.tune_set("coalesce_qubits", 0)
+        N.tune_set("coalesce_qubits", -1)
     for P in plan["passes"]:
         assert P["gthr_ld"][:runs] == list(range(runs)) and P["gthr_st"][:runs] == list(range(runs))
     got = TileEmulator(plan).forward(zero_state(n))
@@ -238,7 +238,7 @@ def test_split_transposes_emulation(dtype, k, elem):
     try:
         full = plan_json(nc.h, dtype=dtype, tile_qubits=k, tlog=8, max_gen=8)
     finally:
-        N.tune_set("half_lds", 1)
+        N.tune_set("half_lds", -1)
     assert all(P["split"] == 0 and P["lds_half"] == 0 for P in full["passes"])
     got = TileEmulator(full, elem=elem).forward(zero_state(n))
     np.testing.assert_allclose(got, oracle.run_gates(n, gates).amp, atol=1e-12)
