@@ -131,7 +131,12 @@ int batch_forward_adjoint(int count, int dtype, const int *n, const int *gate_of
         PlanParams pp = pp0;
         pp.n = ni;
         DMV_TRY(build_plan(f, pp, &B.pf));
-        return build_plan(a, pp, &B.pa);
+        DMV_TRY(build_plan(a, pp, &B.pa));
+        for (size_t p = 0; p + 1 < B.pa.ip_begin.size(); p++)
+            DMV_CHECK_ARG(B.pa.ip_begin[p + 1] - B.pa.ip_begin[p] <= kBatchMaxIp,
+                          "batch circuit %d: %u parameters in one reverse pass (at most %u)", i,
+                          B.pa.ip_begin[p + 1] - B.pa.ip_begin[p], kBatchMaxIp);
+        return DMV_OK;
     };
     {
         const int hw = (int)std::max(1u, std::thread::hardware_concurrency());

@@ -7,18 +7,39 @@
 namespace dmv {
 
 // ---------------------------------------------------------------- batched --
-struct GradSink {
-    static constexpr bool kPerWave = false;
-    double *grad;
-    bool first;  // first tile of the pass: store, later tiles: accumulate
-    __device__ void operator()(const DevOp *op, double v, uint32_t) const
+// Gradient partials of a reverse pass over a circuit's tiles: each wave
+// keeps its own sum per IP op of the pass in LDS (acc[ip_pos][wave], first
+// tile stores, later tiles add, in tile order), so no workgroup barrier is
+// needed per output; after the pass thread 0 adds the waves' sums in wave
+// order (deterministic) into the gradient.
+struct WaveAccSink {
+    static constexpr bool kPerWave = true;
+    double *acc;
+    uint32_t waves;
+    bool first;
+    __device__ void operator()(const DevOp *op, double v, uint32_t wave) const
     {
-        if (first)
-            grad[op->slot] = v;
-        else
-            grad[op->slot] += v;
+        double *a = acc + op->ip_pos * waves + wave;
+        *a = first ? v : *a + v;
     }
 };
+
+// After a reverse pass: the gradient entries of its IP ops (thread 0).
+__device__ __forceinline__ void flush_pass_grads(const DevPass *P, const DevPhase *__restrict__ phases,
+                                                 const DevOp *__restrict__ ops, const double *acc, uint32_t waves,
+                                                 double *g)
+{
+    for (uint32_t ph = P->phase_begin; ph < P->phase_end; ph++)
+        for (uint32_t o = phases[ph].op_begin; o < phases[ph].op_end; o++) {
+            const DevOp *op = ops + o;
+            if (!op_is_ip(op->kind))
+                continue;
+            double s = 0.0;
+            for (uint32_t w = 0; w < waves; w++)
+                s += acc[op->ip_pos * waves + w];
+            g[op->slot] = s;
+        }
+}
 
 // h(x) of the chain observable sum_q a_q z_q + sum_{q<n-1} b_q z_q z_{q+1}.
 __device__ __forceinline__ double chain_h(uint64_t x, uint32_t n, const double *a, const double *b)
@@ -46,6 +67,7 @@ k_batch(const BatchCircuit *__restrict__ circ, const uint32_t *__restrict__ orde
     constexpr int K = RLOG + TLOG, NT = 1 << TLOG;
     __shared__ V lds[1 << K];
     __shared__ double red[NT / 64];
+    __shared__ double acc[kBatchMaxIp * (NT / 64)];
     __shared__ uint32_t s_idx;
     const uint32_t tid = threadIdx.x;
     V *psi = scratch + (uint64_t)blockIdx.x * 2 * slot_amps;
@@ -137,9 +159,12 @@ k_batch(const BatchCircuit *__restrict__ circ, const uint32_t *__restrict__ orde
         __syncthreads();
         for (uint32_t p = C.adj_begin; p < C.adj_end; p++) {
             for (uint64_t t = 0; t < n_tiles; t++)
-                tile_body_adjoint<R, RLOG, TLOG>(psi, lam, passes + p, phases, ops, t, lds, red, GradSink{g, t == 0},
-                                                 kPlainLayout);
+                tile_body_adjoint<R, RLOG, TLOG>(psi, lam, passes + p, phases, ops, t, lds, red,
+                                                 WaveAccSink{acc, NT / 64, t == 0}, kPlainLayout);
             __syncthreads();
+            if (tid == 0)
+                flush_pass_grads(passes + p, phases, ops, acc, NT / 64, g);
+            __syncthreads();  // the next pass's first tile overwrites acc
         }
     }
 }

@@ -30,6 +30,9 @@ int launch_small_adjoint(int dtype, void *psi, void *lam, int n, const DevOp *op
 // Batched persistent launch: one circuit per queue entry.  Circuits with
 // n < k run with psi and lambda in LDS (ops in `sops`, global positions);
 // larger ones run tile passes over per-workgroup scratch states.
+// IP ops (gradient outputs) per reverse pass that the batched kernel keeps
+// per-wave sums for in LDS; the host rejects plans with more.
+constexpr uint32_t kBatchMaxIp = 128;
 struct BatchCircuit {
     uint32_t n, small;
     uint32_t fwd_begin, fwd_end;  // pass range (tile path) or op range (small path)
