@@ -306,7 +306,9 @@ int dmv_circuit_plan_json(dmv_circuit *c, int dtype, int adjoint, int tile_qubit
  * must be the exclusive prefix sum of the circuits' parameter counts
  * (param_offset[0] = 0), so grad holds sum_i (parameters of circuit i)
  * values; anything else returns DMV_ERR_ARG before the device is touched.
- * Circuits have 2 to 24 qubits. */
+ * Circuits have 2 to 24 qubits, and a circuit's reverse sweep may hold at
+ * most 128 parametrised gates in one fused pass (the kernel keeps their
+ * per-wave sums in LDS); a batch with more returns DMV_ERR_ARG. */
 int dmv_batch_forward_adjoint(int count, int dtype, const int *n, const int *gate_offset,
                               const int *gate_kind, const int *gate_control,
                               const int *gate_target, const double *gate_param,

@@ -270,6 +270,19 @@ def test_batch_rejects_bad_offsets_on_host():
     assert "coef_offset[3]" in N.last_error()
 
 
+def test_batch_rejects_too_many_parameters_per_pass_on_host():
+    """The batched kernel keeps per-wave gradient sums for at most 128 IP ops
+    of one reverse pass in LDS; a circuit whose reverse sweep puts more in
+    one pass (200 rotations of one qubit: one hoisted IP group) is refused on
+    the host, before the device is touched."""
+    from damavand_amd.batch import pack_circuits
+    n = 12
+    gates = [(wl.GATE_H, -1, q, ()) for q in range(n)] + [(wl.GATE_RY, -1, 0, (0.01 * k,)) for k in range(200)]
+    b = pack_circuits([(n, gates, np.ones(n), np.ones(n - 1))])
+    assert _batch_call(b) == 1
+    assert "parameters in one reverse pass" in N.last_error()
+
+
 _TUNE_PROBE = r"""
 import ctypes, json, sys
 sys.path.insert(0, sys.argv[1])
