@@ -449,6 +449,9 @@ __device__ __forceinline__ void oop_op(V (&out)[NR], const V (&in)[NR], const De
 }
 
 // The ops [ob, oe) of a phase, alternating between v and a second array.
+// Each op is copied whole into scalar registers first: a few wide scalar
+// loads and one wait, instead of a load and a wait per field as the code
+// reaches it.
 template <class V, class R, int RLOG>
 __device__ __forceinline__ void run_ops(V (&v)[1 << RLOG], const DevOp *__restrict__ ops, uint32_t ob, uint32_t oe,
                                         uint32_t tid)
@@ -456,11 +459,14 @@ __device__ __forceinline__ void run_ops(V (&v)[1 << RLOG], const DevOp *__restri
     V w[1 << RLOG];
     uint32_t o = ob;
     for (; o + 1 < oe; o += 2) {
-        oop_op<V, R>(w, v, ops + o, tid);
-        oop_op<V, R>(v, w, ops + o + 1, tid);
+        const DevOp a = ops[o];
+        oop_op<V, R>(w, v, &a, tid);
+        const DevOp b = ops[o + 1];
+        oop_op<V, R>(v, w, &b, tid);
     }
     if (o < oe) {
-        oop_op<V, R>(w, v, ops + o, tid);
+        const DevOp a = ops[o];
+        oop_op<V, R>(w, v, &a, tid);
         copy_regs(v, w);
     }
 }
@@ -647,7 +653,10 @@ __device__ __forceinline__ void tile_body_adjoint(typename Cplx<R>::T *__restric
         }
         cur = &D->lay;
         for (uint32_t o = D->op_begin; o < D->op_end; o++) {
-            const DevOp *op = ops + o;
+            // the whole op in scalar registers at once (a few wide scalar
+            // loads and one wait instead of one load and wait per field)
+            const DevOp opv = ops[o];
+            const DevOp *op = &opv;
             if (op_is_ip(op->kind)) {
                 if (!op->plain)
                     ip_sums<V, RLOG>(p, l, op, tid, sx, sy, sz);
