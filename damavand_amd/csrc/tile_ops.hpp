@@ -96,9 +96,10 @@ __device__ __forceinline__ void transpose_half(V (&v)[NR], V *lds, const DevLayo
 
 constexpr int highest_bit(int m) { return m >= 16 ? 16 : m >= 8 ? 8 : m >= 4 ? 4 : m >= 2 ? 2 : 1; }
 
-// In-place pair ops of the reverse sweep.  A GEN on registers (p, p ^ M)
-// of psi and lambda at once, so the per-pair coefficients (scalar selects)
-// are chosen once for both states.  f: p holds the target-bit-1 member, so
+// In-place pair ops of the reverse sweep.  A GEN (plain or not: one code
+// path keeps the kernel small, and a plain GEN's selects all pick A) on
+// registers (p, p ^ M) of psi and lambda at once, so the per-pair
+// coefficients (scalar selects) are chosen once for both states.  f: p holds the target-bit-1 member, so
 // the matrix is applied with rows and columns exchanged.  GEN ops carry no
 // controls (plans only make them from uncontrolled gates, and plan
 // validation checks it), so no pair or lane is left out.
@@ -265,36 +266,6 @@ __device__ __forceinline__ bool thread_controls_hold(const DevOp *op, uint32_t t
 {
     const uint32_t ct = op->ctrl_thr;
     return (tid & ct) == ct;
-}
-
-// GEN with the plain flag: every pair (p, p ^ M) in order, no selects.
-template <int M, class V, class R, int NR>
-__device__ __forceinline__ void op_gen_plain_m(V (&v)[NR], const Mat<R> &A)
-{
-    constexpr int HB = highest_bit(M);
-#pragma unroll
-    for (int p = 0; p < NR; p++) {
-        if (p & HB)
-            continue;
-        apply2(v[p], v[p ^ M], A);
-        __builtin_amdgcn_sched_barrier(0);
-    }
-}
-
-template <class V, class R, int NR>
-__device__ __forceinline__ void op_gen_plain(V (&v)[NR], const DevOp *op)
-{
-    const Mat<R> A = load_mat<R>(op->m);
-    switch (op->rmask) {
-#define DMV_PLAIN_CASE(M)                                                                          \
-    case M:                                                                                        \
-        if constexpr (M < NR)                                                                      \
-            op_gen_plain_m<M>(v, A);                                                               \
-        break;
-        DMV_MASK_CASES(DMV_PLAIN_CASE)
-#undef DMV_PLAIN_CASE
-    default: break;
-    }
 }
 
 // ------------------------------------------------ out-of-place op bodies --
@@ -677,9 +648,6 @@ __device__ __forceinline__ void tile_body_adjoint(typename Cplx<R>::T *__restric
             if (op->kind == OP_DIAG) {
                 op_diag<V, R>(p, op, tid, hold);
                 op_diag<V, R>(l, op, tid, hold);
-            } else if (op->plain) {
-                op_gen_plain<V, R>(p, op);
-                op_gen_plain<V, R>(l, op);
             } else if (op->kind == OP_GEN) {
                 op_gen2<V, R>(p, l, op);
             } else {
