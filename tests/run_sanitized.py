@@ -6,7 +6,8 @@ AddressSanitizer and UndefinedBehaviorSanitizer.
 Builds the instrumented copies (``python -m damavand_amd.build --sanitize``:
 host code of libdamavand-gpu with -fsanitize=address,undefined, device code
 unchanged; ``make -C oracle san``), then runs tests/test_host.py,
-tests/test_oracle.py and tests/test_qml_host.py in a child process with
+tests/test_oracle.py, tests/test_qml_host.py and tests/test_tile_emulation.py
+(the planner's plans, checked index by index) in a child process with
 clang's ASan runtime preloaded and DMV_LIBRARY / DMV_ORACLE_LIBRARY pointing
 at the instrumented builds.  Any sanitizer report fails the run
 (halt_on_error); leak checking is off because the Python interpreter itself
@@ -41,7 +42,8 @@ def main() -> int:
     env["DMV_ORACLE_LIBRARY"] = os.path.join(ROOT, "oracle", "_san", "liboracle.so")
     env["ASAN_OPTIONS"] = "detect_leaks=0:halt_on_error=1:abort_on_error=1"
     env["UBSAN_OPTIONS"] = "print_stacktrace=1:halt_on_error=1"
-    tests = [os.path.join(ROOT, "tests", t) for t in ("test_host.py", "test_oracle.py", "test_qml_host.py")]
+    tests = [os.path.join(ROOT, "tests", t)
+             for t in ("test_host.py", "test_oracle.py", "test_qml_host.py", "test_tile_emulation.py")]
     cmd = [sys.executable, "-m", "pytest", "-q", "-p", "no:cacheprovider", "-m", "not gpu"] + tests + args
     print("sanitized run:", " ".join(cmd), flush=True)
     return subprocess.run(cmd, cwd=ROOT, env=env).returncode
