@@ -689,10 +689,17 @@ int build_plan(const std::vector<HostOp> &ops, const PlanParams &pp, Plan *out)
         DMV_CHECK_ARG(op.target >= 0 && op.target < pp.n, "target %d out of range", op.target);
         DMV_CHECK_ARG((op.ctrl & ~all) == 0 && !((op.ctrl >> op.target) & 1), "bad control mask");
     }
+    // With split transposes, a pass keeps one of its qubits above the lane
+    // and swizzle-group bits free of register targets (the split bit): an op
+    // that would make every such qubit a register target waits for the next
+    // pass.  The lowest qubits are always in the tile, so those candidates
+    // are the tile's qubits outside the lowest max(low_bits, swz_bits).
+    const int split_keep = pp.half_lds && pp.tlog >= 7 ? pp.k - std::max(pp.low_bits, pp.swz_bits) - 1 : 64;
+    const uint64_t split_low = (uint64_t(1) << std::max(pp.low_bits, pp.swz_bits)) - 1;
     std::vector<char> done(ops.size(), 0);
     size_t first = 0;
     while (first < ops.size()) {
-        uint64_t Q = lowmask, blocked = 0;
+        uint64_t Q = lowmask, blocked = 0, treg = 0;
         int gen = 0;
         std::vector<HostOp> taken;
         for (size_t i = first; i < ops.size(); i++) {
@@ -707,8 +714,11 @@ int build_plan(const std::vector<HostOp> &ops, const PlanParams &pp, Plan *out)
             }
             const uint64_t nq = Q | qs;
             const bool budget = ops[i].kind != OP_GEN || gen < pp.max_gen_per_pass;
-            if (popcount64(nq) <= pp.k && budget) {
+            const uint64_t nt = op_needs_register(ops[i].kind) ? (treg | (uint64_t(1) << ops[i].target)) : treg;
+            const bool split_ok = popcount64(nt & ~split_low) <= split_keep;
+            if (popcount64(nq) <= pp.k && budget && split_ok) {
                 Q = nq;
+                treg = nt;
                 taken.push_back(ops[i]);
                 done[i] = 1;
                 if (ops[i].kind == OP_GEN)
