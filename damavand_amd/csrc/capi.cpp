@@ -806,8 +806,10 @@ int dmv_circuit_plan_json(dmv_circuit *c, int dtype, int adjoint, int tile_qubit
         pp.rlog = pp.k - pp.tlog;
         pp.half_lds = !adjoint && tune_value("half_lds", 1) != 0 && tile_half_exists(dtype, pp.rlog, pp.tlog) ? 1 : 0;
     }
-    if (max_gen_per_pass > 0)
+    if (max_gen_per_pass > 0) {
         pp.max_gen_per_pass = max_gen_per_pass;
+        pp.gen_auto = 0;
+    }
     std::vector<HostOp> ops = merge_ops(adjoint ? adjoint_ops(c->gates) : forward_ops(c->gates));
     Plan plan;
     DMV_TRY(build_plan(ops, pp, &plan));
@@ -985,8 +987,15 @@ int dmv_circuit_tile_shape(dmv_circuit *c, int dtype, int adjoint, int *tile_qub
         *tile_qubits = pp.k;
     if (threads_log2)
         *threads_log2 = pp.tlog;
-    if (max_gen_per_pass)
+    if (max_gen_per_pass) {
         *max_gen_per_pass = pp.max_gen_per_pass;
+        if (pp.gen_auto && c->n > pp.k) {
+            // chosen per circuit: the budget its forward plan takes
+            Plan plan;
+            DMV_TRY(build_plan(merge_ops(forward_ops(c->gates)), pp, &plan));
+            *max_gen_per_pass = plan.max_gen;
+        }
+    }
     if (nontemporal)
         *nontemporal = pp.nontemporal;
     return DMV_OK;

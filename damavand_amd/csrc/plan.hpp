@@ -56,10 +56,12 @@ struct PlanParams {
     int max_gen_per_pass;  // compute budget of a pass, in general 2x2 ops
     int nontemporal = 0;   // forward passes: streaming (nontemporal) loads and stores
     int half_lds = 0;      // forward passes: split transposes where a pass allows (half-LDS kernels exist)
+    int gen_auto = 0;      // forward plans: build_plan picks max_gen_per_pass (8 or 12) by plan cost
     StateLayout layout = kPlainLayout;  // element map of the states the plan runs on
 };
 
 struct Plan {
+    int max_gen = 0;             // the GEN budget per pass the plan was built with
     std::vector<DevPass> passes;
     std::vector<DevPhase> phases;
     std::vector<DevOp> ops;      // tile-local ops, grouped by phase

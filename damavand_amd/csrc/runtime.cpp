@@ -319,11 +319,13 @@ PlanParams tile_params(int n, int dtype, bool adjoint, StateLayout layout, const
     // slower at 24 qubits, where much of the state stays in the last-level cache
     int tlog = 8, gen = 8, nt = ((uint64_t)amp_bytes(dtype) << n) >= (uint64_t(2) << 30) ? 1 : 0;
     bool chosen = false;  // the shape was measured for this size (table or circuit)
+    bool gen_set = false;  // the GEN budget was given (table, tunable, circuit)
     TuneChoice t;
     if (tune_lookup(tune_key(n, dtype, adjoint ? "adjoint" : "forward"), &t) && t.set()) {
         k = t.k;
         tlog = t.tlog > 0 ? t.tlog : tlog;
         gen = t.max_gen > 0 ? t.max_gen : gen;
+        gen_set = t.max_gen > 0;
         nt = t.nt >= 0 ? t.nt : nt;
         chosen = true;
     }
@@ -334,12 +336,16 @@ PlanParams tile_params(int n, int dtype, bool adjoint, StateLayout layout, const
         tlog = t_set > 0 ? (int)t_set : 8;
         chosen = false;
     }
-    gen = (int)tune_value("max_gen_per_pass", gen);
+    if (tune_value("max_gen_per_pass", -1) > 0) {
+        gen = (int)tune_value("max_gen_per_pass", gen);
+        gen_set = true;
+    }
     nt = (int)tune_value("nontemporal", nt);
     if (pin && pin->set()) {
         k = pin->k;
         tlog = pin->tlog > 0 ? pin->tlog : tlog;
         gen = pin->max_gen > 0 ? pin->max_gen : gen;
+        gen_set = gen_set || pin->max_gen > 0;
         nt = pin->nt >= 0 ? pin->nt : nt;
         chosen = true;
     }
@@ -354,6 +360,8 @@ PlanParams tile_params(int n, int dtype, bool adjoint, StateLayout layout, const
     pp.tlog = tlog;
     pp.rlog = k - tlog;
     pp.max_gen_per_pass = gen;
+    // forward plans choose 8 or 12 per circuit unless a budget was given
+    pp.gen_auto = !adjoint && !gen_set ? 1 : 0;
     pp.nontemporal = nt != 0 && !adjoint ? 1 : 0;
     // forward passes split their transposes where the shape has half-LDS kernels
     pp.half_lds = !adjoint && tune_value("half_lds", 1) != 0 && tile_half_exists(dtype, pp.rlog, pp.tlog) ? 1 : 0;
@@ -398,6 +406,7 @@ int prepare_plan(DevicePlan &dp, uint64_t version, int n, int dtype, bool adjoin
     PlanParams pp = small ? PlanParams{} : tile_params(n, dtype, adjoint, layout, pin);
     const bool same_shape = dp.valid && dp.dtype == dtype && dp.small == small &&
                             (small || (dp.pp.k == pp.k && dp.pp.tlog == pp.tlog && dp.pp.max_gen_per_pass == pp.max_gen_per_pass &&
+                                       dp.pp.gen_auto == pp.gen_auto &&
                                        dp.pp.nontemporal == pp.nontemporal && dp.pp.half_lds == pp.half_lds &&
                                        same_layout(dp.pp.layout, pp.layout)));
     if (same_shape && dp.version == version && dp.epoch == tune_epoch())
