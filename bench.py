@@ -495,7 +495,7 @@ def main():
     ap.add_argument("--stress-layers", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: the state lives on a single MI355X and this build has no "
+        print("bench.py measures one GPU: the state lives on a single MI355X and the library has no "
               "multi-GPU path. Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
