@@ -81,6 +81,32 @@ def test_random_circuit_fused(native, n):
     assert err <= TOL
 
 
+@pytest.mark.parametrize("n,layers", [(20, 6), (22, 6)])
+def test_budget_chosen_per_circuit(native, n, layers):
+    """Random U3 + CNOT/CZ layers: the forward plan takes GEN budget 12 by its
+    pass and phase counts (every pass with split transposes), and matches the
+    oracle; the same circuit with the budget fixed at 8 matches too."""
+    say(f"case GEN budget per circuit n={n} layers={layers}")
+    gates = wl.stress_circuit(n, layers, 1005)
+    c = build_circuit(n, gates)
+    shape = c.tile_shape()
+    assert shape["max_gen_per_pass"] == 12, shape
+    ref = oracle.run_gates(n, gates).amp
+    c.forward()
+    err = np.max(np.abs(c.retrieve_amplitudes_on_host() - ref))
+    say(f"  budget 12: passes={c.plan_stats()[0]} max-abs error {err:.3e}")
+    assert err <= TOL
+    N.tune_set("max_gen_per_pass", 8)
+    try:
+        c8 = build_circuit(n, gates)
+        c8.forward()
+        err8 = np.max(np.abs(c8.retrieve_amplitudes_on_host() - ref))
+        say(f"  budget 8: passes={c8.plan_stats()[0]} max-abs error {err8:.3e}")
+    finally:
+        N.tune_set("max_gen_per_pass", -1)
+    assert err8 <= TOL
+
+
 @pytest.mark.parametrize("n,grid,nt", [(18, 3, 0), (20, 16, 1), (21, 7, 0)])
 def test_pipelined_tile_passes(native, n, grid, nt):
     """The persistent, software-pipelined tile kernel (forced onto a small
