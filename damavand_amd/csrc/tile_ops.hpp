@@ -99,10 +99,11 @@ constexpr int highest_bit(int m) { return m >= 16 ? 16 : m >= 8 ? 8 : m >= 4 ? 4
 // In-place pair ops of the reverse sweep.  A GEN (plain or not: one code
 // path keeps the kernel small, and a plain GEN's selects all pick A) on
 // registers (p, p ^ M) of psi and lambda at once, so the per-pair
-// coefficients (scalar selects) are chosen once for both states.  f: p holds the target-bit-1 member, so
-// the matrix is applied with rows and columns exchanged.  GEN ops carry no
-// controls (plans only make them from uncontrolled gates, and plan
-// validation checks it), so no pair or lane is left out.
+// coefficients (scalar selects) are chosen once for both states.  f: p
+// holds the target-bit-1 member, so the matrix is applied with rows and
+// columns exchanged.  GEN ops carry no controls (plans only make them from
+// uncontrolled gates, and plan validation checks it), so no pair or lane is
+// left out.
 template <int M, class V, class R, int NR>
 __device__ __forceinline__ void op_gen2_m(V (&a)[NR], V (&b)[NR], const Mat<R> &A, uint32_t fl)
 {
