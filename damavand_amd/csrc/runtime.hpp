@@ -52,6 +52,8 @@ struct dmv_state {
     uint64_t stage_cap = 0;       // amplitudes per staged chunk
     double *terms = nullptr;      // Z/ZZ observable terms of dmv_expect_zz (on first use)
     uint64_t cap_terms = 0;
+    void *bounce = nullptr;       // pinned host buffer (two halves) of device-to-host copies (on first use)
+    hipEvent_t bounce_ev[2] = {nullptr, nullptr};
 };
 
 namespace dmv {

@@ -195,7 +195,9 @@ int dmv_apply_matrix(dmv_state *s, const double *m_re, const double *m_im, uint6
 int dmv_apply_gate(dmv_state *s, int kind, int control, int target, const double *params);
 
 /* ---- measurement ---- */
-/* |a_i|^2 for all i into a host buffer of 2^n doubles (circuit.rs:565-589). */
+/* |a_i|^2 for all i into a host buffer of 2^n doubles (circuit.rs:565-589).
+ * This and dmv_state_get copy more than 64 KiB through a 16 MiB pinned host
+ * buffer of the state (allocated on first use, freed with the state). */
 int dmv_probabilities(dmv_state *s, double *host_probabilities);
 /* sum |a_i|^2. */
 int dmv_norm_sqr(dmv_state *s, double *out);
