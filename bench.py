@@ -21,7 +21,8 @@ state (16 GiB) that stays resident in HBM.  Prints one JSON line:
   kernels versus parameter shift with plain fused forwards on the same GPU,
   against the oracle's parameter shift (fixture) and central differences;
 * config1_gpu: the 20-qubit workload on the GPU with its errors against
-  the oracle's results;
+  the oracle's results (timed twice: the second, steady run is reported,
+  the first under first_run);
 * batch: 4096 circuits of 12-18 qubits, forward + adjoint in one launch per
   dtype (complex128, complex64), energies against the oracle;
 * stress: the 33-qubit complex128 circuit (128 GiB), <Z_i> and 10^5 samples.
@@ -387,30 +388,41 @@ def bench_config1_gpu(n, layers, samples, cpu_result):
     c.forward()
     c.measure()
     sync()
-    c.reset_amplitudes()
-    sync()
-    t0 = time.perf_counter()
-    c.forward()
-    sync()
-    t_fwd = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    prob = c.measure()
-    t_prob = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    s = c.state.sample_uniforms(u)
-    t_samp = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    ez = c.expectation_z()
-    t_z = time.perf_counter() - t0
-    out = {"forward_s": t_fwd, "probabilities_s": t_prob, "sample_s": t_samp, "expect_z_s": t_z,
-           "total_s": t_fwd + t_prob + t_samp + t_z}
+
+    def timed_run():
+        c.reset_amplitudes()
+        sync()
+        t0 = time.perf_counter()
+        c.forward()
+        sync()
+        t_fwd = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        prob = c.measure()
+        t_prob = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        s = c.state.sample_uniforms(u)
+        t_samp = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        ez = c.expectation_z()
+        t_z = time.perf_counter() - t0
+        times = {"forward_s": t_fwd, "probabilities_s": t_prob, "sample_s": t_samp, "expect_z_s": t_z,
+                 "total_s": t_fwd + t_prob + t_samp + t_z}
+        return times, prob, s, ez
+
+    # The first timed run still pays a once-per-process cost of the runtime's
+    # second large device-to-host copy (about 7 ms); the second run is the
+    # steady state.  Both are reported, the second as the workload's figures.
+    first, _, _, _ = timed_run()
+    out, prob, s, ez = timed_run()
+    out["first_run"] = first
     if cpu_result is not None:
         amp, p_ref, s_ref, ez_ref = cpu_result
         out["max_abs_err_amplitudes"] = float(np.max(np.abs(c.retrieve_amplitudes_on_host() - amp)))
         out["max_abs_err_probabilities"] = float(np.max(np.abs(prob - p_ref)))
         out["max_abs_err_expect_z"] = float(np.max(np.abs(ez - ez_ref)))
         out["samples_equal"] = int(np.sum(s == s_ref))
-    log(f"[bench] gpu 20q workload: {out['total_s'] * 1e3:.1f} ms")
+    log(f"[bench] gpu 20q workload: {out['total_s'] * 1e3:.1f} ms (first run in the process "
+        f"{first['total_s'] * 1e3:.1f} ms)")
     return out
 
 
