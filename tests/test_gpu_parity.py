@@ -197,6 +197,27 @@ def test_layered_workload_20q(native):
     assert np.max(np.abs(c.retrieve_amplitudes_on_host() - st.amp)) <= TOL
 
 
+def test_host_copies_in_pieces(native):
+    """Probabilities and amplitudes of 22- and 21-qubit states (32 and 16 MiB
+    per array, so four and two pieces through each state's pinned buffer,
+    alternating its halves), read twice into fresh arrays, against the
+    oracle; the two states keep their own buffers."""
+    rng = np.random.default_rng(2201)
+    runs = []
+    for n in (22, 21):
+        say(f"case host copies in pieces {n}q")
+        gates = random_gates(rng, n, 60)
+        c = build_circuit(n, gates)
+        c.forward()
+        runs.append((c, oracle.run_gates(n, gates)))
+    for rep in range(2):
+        for c, st in runs:
+            e_p = np.max(np.abs(c.measure() - st.probabilities()))
+            e_amp = np.max(np.abs(c.retrieve_amplitudes_on_host() - st.amp))
+            say(f"  {c.num_qubits}q read {rep}: probabilities {e_p:.3e} amplitudes {e_amp:.3e}")
+            assert e_p <= TOL and e_amp <= TOL
+
+
 def test_observables_are_skipped(native):
     c = Circuit(3, apply_method="gpu")
     c.add_hadamard_gate(0)
