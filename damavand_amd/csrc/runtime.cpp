@@ -499,10 +499,10 @@ void state_destroy(dmv_state *s)
     if (s->bounce) {
         (void)hipStreamSynchronize(stream());
         (void)hipHostFree(s->bounce);
-        for (hipEvent_t &e : s->bounce_ev)
-            if (e)
-                (void)hipEventDestroy(e);
     }
+    for (hipEvent_t &e : s->bounce_ev)
+        if (e)
+            (void)hipEventDestroy(e);
     delete s;
 }
 
@@ -545,9 +545,13 @@ static int bounce_d2h(dmv_state *s, void *dst, const void *src, uint64_t bytes)
     if (bytes <= (uint64_t(64) << 10))
         return copy_d2h(dst, src, bytes);
     if (!s->bounce) {
-        HIPCALL(hipHostMalloc(&s->bounce, 2 * kBounceHalf, hipHostMallocDefault));
+        // events first, then the buffer: the state holds either all three or none
         for (hipEvent_t &e : s->bounce_ev)
-            HIPCALL(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            if (!e)
+                HIPCALL(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        void *b = nullptr;
+        HIPCALL(hipHostMalloc(&b, 2 * kBounceHalf, hipHostMallocDefault));
+        s->bounce = b;
     }
     char *buf[2] = {(char *)s->bounce, (char *)s->bounce + kBounceHalf};
     const uint64_t pieces = (bytes + kBounceHalf - 1) / kBounceHalf;
