@@ -4,7 +4,8 @@ Bars: amplitudes, probabilities and expectations within 1e-12 max-abs in
 complex128; gradients within 1e-10 of the oracle's parameter-shift
 gradient; samples exactly equal to the oracle's for the same uniforms (or
 within 1e-12 of a cumulative boundary), and chi-square consistent for seeded
-draws.  complex64 results are held to 1e-4 and the error is printed.
+draws.  complex64 results are held to 1e-4 (reverse-sweep energies and
+gradients to 1e-5 of the complex128 adjoint) and the error is printed.
 """
 import ctypes
 import math
