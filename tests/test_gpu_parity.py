@@ -400,6 +400,25 @@ def test_adjoint_complex64(native, n, layers, k):
     assert ee <= 1e-5 and eg <= 1e-5
 
 
+def test_adjoint_narrow_tile(native):
+    """The reverse sweep with 4 amplitudes per thread and state (k = 10, a
+    tunable) against the default 8 (k = 11) on an 18-qubit ansatz."""
+    say("case adjoint k=10 against k=11, n=18")
+    n = 18
+    gates, a, b = wl.ansatz_circuit(n, 3, 3300)
+    pairs = wl.chain_pairs(n)
+    e_ref, g_ref = build_circuit(n, gates).gradient(a, pairs, b)
+    c = build_circuit(n, gates)
+    N.tune_set("adjoint_tile_qubits", 10)
+    try:
+        e, g = c.gradient(a, pairs, b)
+    finally:
+        N.tune_set("adjoint_tile_qubits", -1)
+    ee, eg = abs(e - e_ref), float(np.max(np.abs(g - g_ref)))
+    say(f"  energy {ee:.3e} gradient max-abs {eg:.3e} over {g.size} parameters")
+    assert ee <= 1e-12 and eg <= 1e-12
+
+
 def test_adjoint_closed_form(native):
     # RY(t)|0>, H = Z: E = cos t, dE/dt = -sin t
     for t in (0.0, 0.4, 2.0, 4.5):
