@@ -463,6 +463,38 @@ def test_complex64_forward(native):
         assert err < 1e-4
 
 
+@pytest.mark.parametrize("k,tlog", [(10, 8), (10, 7), (12, 9)])
+def test_batch_tile_shapes(native, k, tlog):
+    """The batched kernel's other tile shapes (tunables): 4 amplitudes per
+    thread and state on 256 threads, 8 on 128 and 8 on 512, on circuits on
+    both sides of the tile size (LDS path below it, tile path from it up),
+    against the default shape in complex128 and, for complex64, against the
+    complex128 results."""
+    say(f"case batch shape k={k} tlog={tlog}")
+    from damavand_amd import batch_forward_adjoint
+    circuits = wl.batch_workload(16, 4500 + k + tlog, n_range=(8, 14), layer_range=(2, 3))
+    e_ref, g_ref = batch_forward_adjoint(circuits, dtype="complex128")
+    N.tune_set("batch_tile_qubits", k)
+    N.tune_set("batch_threads_log2", tlog)
+    try:
+        e128, g128 = batch_forward_adjoint(circuits, dtype="complex128")
+        e64, g64 = batch_forward_adjoint(circuits, dtype="complex64")
+    finally:
+        N.tune_set("batch_tile_qubits", -1)
+        N.tune_set("batch_threads_log2", -1)
+    worst = {"complex128": [0.0, 0.0], "complex64": [0.0, 0.0]}
+    for i in range(len(circuits)):
+        for key, e, g in (("complex128", e128, g128), ("complex64", e64, g64)):
+            worst[key][0] = max(worst[key][0], abs(e[i] - e_ref[i]))
+            if g_ref[i].size:
+                worst[key][1] = max(worst[key][1], float(np.max(np.abs(g[i] - g_ref[i]))))
+    say(f"  against the default shape: complex128 energy {worst['complex128'][0]:.3e} gradient "
+        f"{worst['complex128'][1]:.3e}; complex64 energy {worst['complex64'][0]:.3e} gradient "
+        f"{worst['complex64'][1]:.3e}")
+    assert max(worst["complex128"]) <= 1e-12
+    assert max(worst["complex64"]) <= 1e-5
+
+
 def test_batch_matches_single_circuits(native):
     say("case batch")
     circuits = wl.batch_workload(24, 4242, n_range=(3, 14), layer_range=(1, 4))
