@@ -463,6 +463,31 @@ def test_complex64_forward(native):
         assert err < 1e-4
 
 
+@pytest.mark.parametrize("k,tlog", [(13, 8), (12, 8), (11, 8), (10, 6), (9, 6)])
+def test_complex64_tile_shapes(native, k, tlog):
+    """Every complex64 forward tile with whole-tile transposes (32, 16 and 8
+    amplitudes per thread on 256 threads, 16 and 8 on 64), forced by the
+    tunables, against the FP64 oracle."""
+    n = 20
+    say(f"case complex64 tile shape n={n} k={k} tlog={tlog}")
+    rng = np.random.default_rng(600 + k + tlog)
+    gates = random_gates(rng, n, 250)
+    c = build_circuit(n, gates, dtype="complex64")
+    knobs = dict(tile_qubits=k, tile_threads_log2=tlog, half_lds=0)
+    for key, v in knobs.items():
+        N.tune_set(key, v)
+    try:
+        shape = c.tile_shape()
+        c.forward()
+    finally:
+        for key in knobs:
+            N.tune_set(key, -1)
+    assert (shape["tile_qubits"], shape["threads_log2"]) == (k, tlog), shape
+    err = np.max(np.abs(c.retrieve_amplitudes_on_host() - oracle.run_gates(n, gates).amp))
+    say(f"  passes={c.plan_stats()[0]} max-abs error vs FP64 oracle {err:.3e}")
+    assert err < 1e-4
+
+
 @pytest.mark.parametrize("k,tlog", [(10, 8), (10, 7), (12, 9)])
 def test_batch_tile_shapes(native, k, tlog):
     """The batched kernel's other tile shapes (tunables): 4 amplitudes per
