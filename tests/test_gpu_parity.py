@@ -132,6 +132,7 @@ TILE_CASES = [
     (20, 13, 9, 8, {}), (19, 12, 9, 4, {}), (20, 11, 8, 40, {}), (22, 12, 8, 2, {}), (20, 11, 6, 8, {}),
     (20, 10, 6, 16, {}), (21, 12, 8, 8, {"tile_xcd_log2": 0}), (20, 12, 8, 8, {"coalesce_qubits": 5}),
     (20, 12, 8, 8, {"register_renaming": 0}), (21, 11, 7, 8, {}), (21, 11, 7, 6, {"tile_xcd_log2": 0}),
+    (20, 10, 8, 8, {}), (20, 12, 8, 8, {"half_lds": 0}),
 ]
 
 
@@ -139,7 +140,8 @@ TILE_CASES = [
 def test_tile_shapes(native, n, k, tlog, gen, extra):
     """Every tile shape and launch option the tuner may pick (64- to
     512-thread workgroups, 10- to 13-qubit tiles, compute budgets per pass,
-    XCD-contiguous tile order, wider coalescing runs, renaming off)."""
+    XCD-contiguous tile order, wider coalescing runs, renaming off, whole-tile
+    transposes in place of split ones)."""
     say(f"case tile shape n={n} k={k} tlog={tlog} gen={gen} {extra}")
     rng = np.random.default_rng(400 + n + k)
     gates = random_gates(rng, n, 300)
