@@ -374,31 +374,37 @@ def test_adjoint_vs_parameter_shift(native, n, layers):
     assert c.retrieve_amplitudes_on_host()[0] == 1.0
 
 
-@pytest.mark.parametrize("n,layers,k", [(12, 2, 0), (18, 2, 0), (20, 2, 12)])
-def test_adjoint_complex64(native, n, layers, k):
+@pytest.mark.parametrize("n,layers,knobs", [
+    (12, 2, {}), (18, 2, {}), (20, 2, {"adjoint_tile_qubits": 12}),
+    (18, 2, {"adjoint_tile_qubits": 10, "tile_qubits": 10, "tile_threads_log2": 6})])
+def test_adjoint_complex64(native, n, layers, knobs):
     """The reverse sweep in complex64 (FP32 amplitudes, FP64 sums) against
     the complex128 adjoint, which at 12 qubits is itself checked against the
     oracle's parameter shift.  The default tiles are 8 amplitudes per thread
-    and state with 64 threads (12 qubits) or 256 threads (18); k = 12 selects
-    16 amplitudes per thread."""
-    say(f"case adjoint complex64 n={n} layers={layers} k={k or 'default'}")
+    and state with 64 threads (12 qubits) or 256 threads (18); the tunables
+    select 16 amplitudes per thread on 256 threads, and on 64."""
+    say(f"case adjoint complex64 n={n} layers={layers} {knobs}")
     gates, a, b = wl.ansatz_circuit(n, layers, 3100 + n)
     pairs = wl.chain_pairs(n)
     e_ref, g_ref = build_circuit(n, gates).gradient(a, pairs, b)
     if n <= 12:
         assert np.max(np.abs(g_ref - _param_shift(n, gates, a, pairs, b))) <= 1e-10
     c64 = build_circuit(n, gates, dtype="complex64")
-    if k:
-        N.tune_set("adjoint_tile_qubits", k)
+    for key, v in knobs.items():
+        N.tune_set(key, v)
     try:
+        shape = c64.tile_shape(adjoint=True)
         e, g = c64.gradient(a, pairs, b)
     finally:
-        if k:
-            N.tune_set("adjoint_tile_qubits", -1)
+        for key in knobs:
+            N.tune_set(key, -1)
     ee, eg = abs(e - e_ref), float(np.max(np.abs(g - g_ref)))
     say(f"  complex64 vs complex128: energy {ee:.3e} gradient max-abs {eg:.3e} over {g.size} parameters "
         f"(largest |g| {np.max(np.abs(g_ref)):.3e})")
     assert g.shape == g_ref.shape
+    if "adjoint_tile_qubits" in knobs:
+        assert shape["tile_qubits"] == knobs["adjoint_tile_qubits"], shape
+        assert shape["threads_log2"] == knobs.get("tile_threads_log2", 8), shape
     assert ee <= 1e-5 and eg <= 1e-5
 
 
