@@ -196,8 +196,9 @@ int dmv::circuit_adjoint(dmv_circuit *c, dmv_state *psi, dmv_state *lam, const O
         for (uint32_t p = 0; p < n_pass; p++) {
             const uint64_t row0 = dp.plan.ip_begin[p] - pending;
             prof_begin(DMV_KC_ADJOINT);
-            DMV_TRY(launch_tile_adjoint(psi->dtype, dp.pp.rlog, dp.pp.tlog, psi->amp, lam->amp, c->n, psi->layout, dp.d_passes,
-                                        dp.d_phases, dp.d_ops, p, c->d_partials + row0 * per_ip));
+            DMV_TRY(launch_tile_adjoint(psi->dtype, dp.pp.rlog, dp.pp.tlog, dp.pp.nontemporal != 0, psi->amp, lam->amp,
+                                        c->n, psi->layout, dp.d_passes, dp.d_phases, dp.d_ops, p,
+                                        c->d_partials + row0 * per_ip));
             prof_end(DMV_KC_ADJOINT, 4.0 * (double)psi->bytes);
             const uint32_t done = dp.plan.ip_begin[p + 1];
             if (p + 1 == n_pass || dp.plan.ip_begin[p + 2] - pending > rows) {

@@ -18,7 +18,7 @@ bool tile_half_exists(int dtype, int rlog, int tlog);
 // Reverse-sweep tile pass over psi and lambda; the IP ops of the pass write
 // one partial per (IP op, tile, wave) into
 // partials[(ip_pos * tiles + tile) * waves + wave].
-int launch_tile_adjoint(int dtype, int rlog, int tlog, void *psi, void *lam, int n, StateLayout L,
+int launch_tile_adjoint(int dtype, int rlog, int tlog, bool nt, void *psi, void *lam, int n, StateLayout L,
                         const DevPass *passes, const DevPhase *phases, const DevOp *ops,
                         uint32_t pass_idx, double *partials);
 // Every op of a list applied to a whole state held in one workgroup's LDS.

@@ -314,10 +314,15 @@ PlanParams tile_params(int n, int dtype, bool adjoint, StateLayout layout, const
     // defaults: 256 threads; k = 12 (complex128) / 13 (complex64) forward,
     // 11 for reverse sweeps, which hold psi and lambda in registers together
     int k = dtype == DMV_C64 ? (adjoint ? 11 : 13) : (adjoint ? 11 : 12);
-    // streaming (nontemporal) loads and stores from 2 GiB states up: measured
-    // 1.5-3.5% faster per pass at 27, 30 and 33 qubits (complex128), and
-    // slower at 24 qubits, where much of the state stays in the last-level cache
-    int tlog = 8, gen = 8, nt = ((uint64_t)amp_bytes(dtype) << n) >= (uint64_t(2) << 30) ? 1 : 0;
+    // streaming (nontemporal) loads and stores: forward passes from 2 GiB
+    // states up (measured 1.5-3.5% faster per pass at 27, 30 and 33 qubits,
+    // complex128, and slower at 24 qubits, where much of the state stays in
+    // the last-level cache); reverse sweeps once psi and lambda together take
+    // 512 MiB, twice that cache (measured 1% faster per pass at 24 qubits,
+    // 1-3% at 27 and 3.5% at 30)
+    const uint64_t state_bytes = (uint64_t)amp_bytes(dtype) << n;
+    int tlog = 8, gen = 8;
+    int nt = adjoint ? (2 * state_bytes >= (uint64_t(512) << 20) ? 1 : 0) : (state_bytes >= (uint64_t(2) << 30) ? 1 : 0);
     bool chosen = false;  // the shape was measured for this size (table or circuit)
     bool gen_set = false;  // the GEN budget was given (table, tunable, circuit)
     TuneChoice t;
@@ -362,7 +367,7 @@ PlanParams tile_params(int n, int dtype, bool adjoint, StateLayout layout, const
     pp.max_gen_per_pass = gen;
     // forward plans choose 8 or 12 per circuit unless a budget was given
     pp.gen_auto = !adjoint && !gen_set ? 1 : 0;
-    pp.nontemporal = nt != 0 && !adjoint ? 1 : 0;
+    pp.nontemporal = nt != 0 ? 1 : 0;
     // forward passes split their transposes where the shape has half-LDS kernels
     pp.half_lds = !adjoint && tune_value("half_lds", 1) != 0 && tile_half_exists(dtype, pp.rlog, pp.tlog) ? 1 : 0;
     return pp;

@@ -600,7 +600,8 @@ __device__ __forceinline__ void ip_sums(const V (&p)[1 << RLOG], const V (&l)[1 
 // are applied in place, to psi and then to lambda: an out-of-place loop over
 // two states (four register arrays, or three with one run of ops per state
 // between IP ops) needed more registers or twice the code and ran slower.
-template <class R, int RLOG, int TLOG, class Sink>
+// STREAM: streaming (nontemporal) loads and stores of both states.
+template <class R, int RLOG, int TLOG, bool STREAM = false, class Sink>
 __device__ __forceinline__ void tile_body_adjoint(typename Cplx<R>::T *__restrict__ psi,
                                                   typename Cplx<R>::T *__restrict__ lam, const DevPass *P,
                                                   const DevPhase *__restrict__ phases,
@@ -613,8 +614,8 @@ __device__ __forceinline__ void tile_body_adjoint(typename Cplx<R>::T *__restric
     const uint64_t g = tile_base<K>(P, tile);
     const uint64_t gl = g | thread_global_offset<TLOG>(P->gthr_ld, tid);
     V p[NR], l[NR];
-    load_tile<false>(p, psi, gl, P->greg_ld, L);
-    load_tile<false>(l, lam, gl, P->greg_ld, L);
+    load_tile<STREAM>(p, psi, gl, P->greg_ld, L);
+    load_tile<STREAM>(l, lam, gl, P->greg_ld, L);
     const DevLayout *cur = &P->lay_ld;
     double sx = 0.0, sy = 0.0, sz = 0.0;  // sums of the current IP group
     for (uint32_t ph = P->phase_begin; ph < P->phase_end; ph++) {
@@ -662,8 +663,8 @@ __device__ __forceinline__ void tile_body_adjoint(typename Cplx<R>::T *__restric
         transpose<V, NR, TLOG>(l, lds, cur, P->st_src_lswz, &P->lay_st, P->st_dst_lswz, tid);
     }
     const uint64_t gs = g | thread_global_offset<TLOG>(P->gthr_st, tid);
-    store_tile<false>(p, psi, gs, P->greg_st, L);
-    store_tile<false>(l, lam, gs, P->greg_st, L);
+    store_tile<STREAM>(p, psi, gs, P->greg_st, L);
+    store_tile<STREAM>(l, lam, gs, P->greg_st, L);
 }
 
 // ------------------------------------------------------ whole-state in LDS --

@@ -427,6 +427,32 @@ def test_adjoint_narrow_tile(native):
     assert ee <= 1e-12 and eg <= 1e-12
 
 
+@pytest.mark.parametrize("dtype,knobs", [("complex128", {}), ("complex64", {}),
+                                         ("complex64", {"adjoint_tile_qubits": 12})])
+def test_adjoint_streaming(native, dtype, knobs):
+    """Streaming (nontemporal) loads and stores in the reverse sweep change
+    only the cache policy: energy and gradient are bit-identical to the
+    cached form."""
+    say(f"case adjoint streaming {dtype} {knobs}")
+    n = 20
+    gates, a, b = wl.ansatz_circuit(n, 2, 3400)
+    pairs = wl.chain_pairs(n)
+    out = []
+    for nt in (0, 1):
+        c = build_circuit(n, gates, dtype=dtype)
+        for key, v in dict(knobs, nontemporal=nt).items():
+            N.tune_set(key, v)
+        try:
+            assert c.tile_shape(adjoint=True)["nontemporal"] == nt
+            out.append(c.gradient(a, pairs, b))
+        finally:
+            for key in dict(knobs, nontemporal=nt):
+                N.tune_set(key, -1)
+    (e0, g0), (e1, g1) = out
+    say(f"  energy {e0!r} / {e1!r}, gradients equal: {np.array_equal(g0, g1)}")
+    assert e0 == e1 and np.array_equal(g0, g1)
+
+
 def test_adjoint_closed_form(native):
     # RY(t)|0>, H = Z: E = cos t, dE/dt = -sin t
     for t in (0.0, 0.4, 2.0, 4.5):
