@@ -56,6 +56,7 @@ struct BatchLaunch {
     double *grad;
     uint32_t grid;
     int rlog, tlog;       // tile shape: 2^rlog amplitudes per thread, 2^tlog threads
+    bool nt;              // streaming loads and stores of the scratch states (default shape only)
 };
 // Whether the batched kernel is built for this tile shape.
 bool batch_kernel_exists(int dtype, int rlog, int tlog);

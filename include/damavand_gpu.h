@@ -345,8 +345,9 @@ int dmv_synchronize(void);
  * "tile_threads_log2" (6..9), "max_gen_per_pass" (default: 8 or 12, chosen per
  * circuit for forward plans by their pass and phase counts; 8 for reverse
  * sweeps), "nontemporal" (0/1: streaming loads and stores; default 1 for
- * forward passes over states of 2 GiB and more, and for reverse sweeps
- * whose two states together take 512 MiB and more),
+ * forward passes over states of 2 GiB and more, for reverse sweeps whose
+ * two states together take 512 MiB and more, and for batches whose scratch
+ * states take 512 MiB and more),
  * "gate_pairs_per_thread" (1, 2, 4), "batch_tile_qubits",
  * "batch_threads_log2", "batch_wgs_per_cu", "graph" (0/1: replay forward
  * passes from a captured hipGraph), "half_lds" (0/1, default 1: forward

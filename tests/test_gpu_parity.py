@@ -554,6 +554,27 @@ def test_batch_tile_shapes(native, k, tlog):
     assert max(worst["complex64"]) <= 1e-5
 
 
+def test_batch_streaming(native):
+    """Streaming loads and stores of the batched kernel's scratch states
+    (the default from 512 MiB of scratch; forced here on a small batch)
+    change only the cache policy: bit-identical energies and gradients."""
+    say("case batch streaming")
+    from damavand_amd import batch_forward_adjoint
+    circuits = wl.batch_workload(24, 4600, n_range=(8, 14), layer_range=(2, 3))
+    for dtype in ("complex128", "complex64"):
+        out = []
+        for nt in (0, 1):
+            N.tune_set("nontemporal", nt)
+            try:
+                out.append(batch_forward_adjoint(circuits, dtype=dtype))
+            finally:
+                N.tune_set("nontemporal", -1)
+        (e0, g0), (e1, g1) = out
+        same = np.array_equal(e0, e1) and all(np.array_equal(a, b) for a, b in zip(g0, g1))
+        say(f"  {dtype}: bit-identical {same}")
+        assert same
+
+
 def test_batch_matches_single_circuits(native):
     say("case batch")
     circuits = wl.batch_workload(24, 4242, n_range=(3, 14), layer_range=(1, 4))
